@@ -1,0 +1,140 @@
+/*
+ * nans_gpu.h -- C ABI of the MI355X (gfx950) simulation step.
+ *
+ * Drop-in for the physics block of the reference's plugin entry point
+ * SimUpdateAndRender (code/nans.h:396-397, code/nans.cpp:1719-1762), which the
+ * platform layer binds with dlsym (code/sdl_nans.cpp:399-407).  The step is the
+ * reference's IntegrateForces -> DetectCollisions -> SolveConstraints ->
+ * IntegrateVelocities (code/nans.cpp:1758-1762) followed by the Model rebuild
+ * that the reference performs in its render section (code/nans.cpp:1870-1881,
+ * 1913-1941) and that feeds the next step's box vertices.
+ *
+ * Conventions
+ *   - Every call returns int: NANS_OK (0) or a negative NANS_ERR_* code; the
+ *     message of the last failure on the calling thread is nans_last_error().
+ *     Nothing prints and nothing aborts (the reference only printf's,
+ *     code/nans.cpp:752,1109,1324).
+ *   - Vector arrays are structure-of-arrays component planes: a [3][n] array
+ *     holds all x, then all y, then all z.
+ *   - Calls are ordered on the context's stream and synchronise before they
+ *     return.
+ *   - One thread drives a context; a context owns one device.
+ */
+#ifndef NANS_GPU_H
+#define NANS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NANS_OK 0
+#define NANS_ERR_INVALID (-1)   /* bad argument */
+#define NANS_ERR_HIP (-2)       /* HIP runtime error */
+#define NANS_ERR_CAPACITY (-3)  /* input exceeds a documented capacity */
+#define NANS_ERR_STATE (-4)     /* call not valid in the context's state */
+
+/* Shape kinds of the narrowphase entry point and of bodies. */
+#define NANS_SHAPE_BOX 0
+#define NANS_SHAPE_SPHERE 1
+
+/* Pair kinds, numbered as collision_type (code/nans.h:73-89). */
+#define NANS_CC 0
+#define NANS_CS 1
+#define NANS_CF 2
+#define NANS_SS 3
+#define NANS_SF 4
+
+/* GJK outcome (evolve_result, code/nans.h:91-96, plus the 65-call cap). */
+#define NANS_GJK_NO 0
+#define NANS_GJK_FOUND 1
+#define NANS_GJK_CAP 2
+
+/* EPA outcome per pair. */
+#define NANS_EPA_NONE 0       /* EPA not run */
+#define NANS_EPA_CONTACT 1    /* contact (code/nans.cpp:828-866) */
+#define NANS_EPA_BARY 2       /* |barycentric| > 1 -> no contact (code/nans.cpp:836-840) */
+#define NANS_EPA_NONFINITE 3  /* NaN/inf barycentric -> no contact (code/nans.cpp:842-846) */
+#define NANS_EPA_ITER 4       /* 65 iterations without convergence (code/nans.cpp:903) */
+#define NANS_EPA_EMPTY 5      /* every face removed; the reference indexes an empty vector */
+#define NANS_EPA_OVERFLOW 6   /* polytope beyond 72 vertices / 192 faces / 64 horizon
+                                 edges: flagged and counted, never silently wrong */
+
+/* Parameters.  nans_default_params() fills the reference's constants:
+ * 70 solver iterations (code/nans.cpp:1176), beta 0.3 (:1152), restitution 0.1
+ * (:1154), friction 0.1 (:1159), damping GLOBAL_FRICTION 1.5 (code/nans.h:46),
+ * gravity 9.81 (code/nans.cpp:68), 64 GJK / EPA iterations and EPA tolerance
+ * 0.001 (code/nans.h:54-56), the floor box of code/nans.cpp:1666-1678 with
+ * Model scale (100, 1, 100) (code/nans.cpp:1878-1879), dt 1/60. */
+typedef struct nans_params {
+    int32_t solver_iterations;
+    float beta;
+    float restitution;
+    float friction;
+    float damping;
+    float gravity;
+    float dt;
+    int32_t max_gjk_iterations;
+    int32_t max_epa_iterations;
+    float epa_tolerance;
+    int32_t fp64_islands;     /* 0: reference arithmetic (default).  1: GJK's
+                                 termination length and EPA's final face choice
+                                 and depth in binary64. */
+    float floor_pos[3];
+    float floor_scale[3];
+    float floor_mass;
+    float floor_moi;
+    int32_t device;           /* HIP device ordinal */
+} nans_params;
+
+typedef struct nans_ctx nans_ctx;
+
+/* Shapes of the narrowphase entry point (device pointers, n entries each). */
+typedef struct nans_shapes_soa {
+    const int32_t *type;   /* NANS_SHAPE_* */
+    const float *pos;      /* [3][n] centre; also the GJK start direction */
+    const float *quat;     /* [4][n] unit quaternion w,x,y,z (boxes) */
+    const float *half;     /* [3][n] box half-extents; spheres: plane 0 = radius */
+} nans_shapes_soa;
+
+/* Narrowphase outputs (device pointers; any may be NULL). */
+typedef struct nans_narrow_out {
+    int32_t *gjk;          /* NANS_GJK_* */
+    int32_t *contact;      /* 1 if EPA produced a contact */
+    float *normal;         /* [3][n] contact normal (-face normal, code/nans.cpp:851) */
+    double *depth;         /* EPA distance of the chosen face (binary64 in fp64_islands mode) */
+    float *point_a;        /* [3][n] contact point on A (code/nans.cpp:848-855) */
+    float *point_b;        /* [3][n] contact point on B (code/nans.cpp:862-865) */
+    int32_t *status;       /* NANS_EPA_* */
+} nans_narrow_out;
+
+/* Fill p with the reference's constants (see nans_params). */
+void nans_default_params(nans_params *p);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *nans_last_error(void);
+
+/* Create / destroy a context on p->device (p may be NULL for the defaults).
+ * Replaces the reference's one-time Init guarded by Memory->IsInitialized
+ * (code/nans.cpp:1721-1729). */
+int nans_create(nans_ctx **out, const nans_params *p);
+int nans_destroy(nans_ctx *ctx);
+
+/* Replace the parameters (dt, solver constants ...) used by later calls. */
+int nans_set_params(nans_ctx *ctx, const nans_params *p);
+
+/* Order later calls on this hipStream_t (NULL: the null stream). */
+int nans_set_stream(nans_ctx *ctx, void *hip_stream);
+
+/* CheckCollision (code/nans.cpp:906-966) on n independent pairs: GJK then EPA.
+ * Box corners come from Model = [R(q) * 2h | centre] applied to (+-0.5, +-0.5,
+ * +-0.5, 1) in the corner order of code/nans.cpp:399-406.  If n_overflow is not
+ * NULL it receives the number of NANS_EPA_OVERFLOW pairs. */
+int nans_narrowphase(nans_ctx *ctx, int64_t n, const nans_shapes_soa *a, const nans_shapes_soa *b,
+                     const nans_narrow_out *out, int64_t *n_overflow);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

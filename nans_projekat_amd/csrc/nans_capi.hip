@@ -1,0 +1,252 @@
+// nans_capi.hip -- context management and the narrowphase entry point of the
+// C ABI declared in include/nans_gpu.h.
+#include "nans_common.hpp"
+
+#include <new>
+
+namespace nans {
+
+static thread_local char g_error[1024];
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof(g_error), fmt, ap);
+    va_end(ap);
+}
+
+struct NarrowIn {
+    const int32_t *type;
+    const float *pos, *quat, *half;
+};
+
+struct NarrowOutDev {
+    int32_t *gjk, *contact, *status;
+    float *normal, *pa, *pb;
+    double *depth;
+};
+
+__device__ inline Shape load_shape(const NarrowIn &s, int64_t i, int64_t n)
+{
+    Shape sh;
+    sh.kind = s.type[i];
+    sh.c = mk3(s.pos[i], s.pos[n + i], s.pos[2 * n + i]);
+    V3 h = mk3(s.half[i], s.half[n + i], s.half[2 * n + i]);
+    sh.r = h.x;
+    if (sh.kind == SHAPE_BOX) {
+        box_from_quat(sh.c, s.quat[i], s.quat[n + i], s.quat[2 * n + i], s.quat[3 * n + i], h, sh.v);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sh.v[k] = mk3(0.0f, 0.0f, 0.0f);
+    }
+    return sh;
+}
+
+// One lane per pair for GJK; the wave then runs EPA for each of its lanes whose
+// GJK found an intersection, one pair at a time, with the polytope in LDS.
+__global__ __launch_bounds__(256) void k_narrowphase(int64_t n, NarrowIn a, NarrowIn b, NarrowParams p,
+                                                     NarrowOutDev o, unsigned long long *overflow)
+{
+    __shared__ EpaLds lds[4];
+    EpaLds &L = lds[threadIdx.x >> 6];
+    const int lane = lane_id();
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool active = i < n;
+
+    Shape A, B;
+    int g = GJK_NO, calls = 0;
+    Simplex S;
+    S.n = 0;
+    if (active) {
+        A = load_shape(a, i, n);
+        B = load_shape(b, i, n);
+        g = gjk(A, B, p, S, &calls);
+    }
+    int status = EPA_NONE;
+    V3 pa = mk3(0.0f, 0.0f, 0.0f), pb = pa, nrm = pa;
+    double depth = 0.0;
+
+    uint64_t m = ballot(active && g == GJK_FOUND);
+    while (m) {
+        const int src = ffs64(m);
+        m &= m - 1;
+        if (lane == src) {
+            shape_store(L.sa, A);
+            shape_store(L.sb, B);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                st3(L.vp, k, S.s[k].P);
+                st3(L.va, k, S.s[k].A);
+                st3(L.vb, k, S.s[k].B);
+            }
+        }
+        wave_sync();
+        EpaResult r = epa_wave(L, p);
+        if (lane == src) {
+            status = r.status;
+            pa = r.pa;
+            pb = r.pb;
+            nrm = r.n;
+            depth = r.depth;
+        }
+        wave_sync();
+    }
+    if (!active) return;
+    if (o.gjk) o.gjk[i] = g;
+    if (o.contact) o.contact[i] = status == EPA_CONTACT ? 1 : 0;
+    if (o.status) o.status[i] = status;
+    if (o.normal) { o.normal[i] = nrm.x; o.normal[n + i] = nrm.y; o.normal[2 * n + i] = nrm.z; }
+    if (o.pa) { o.pa[i] = pa.x; o.pa[n + i] = pa.y; o.pa[2 * n + i] = pa.z; }
+    if (o.pb) { o.pb[i] = pb.x; o.pb[n + i] = pb.y; o.pb[2 * n + i] = pb.z; }
+    if (o.depth) o.depth[i] = depth;
+    if (status == EPA_OVERFLOW) atomicAdd(overflow, 1ull);
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace nans
+
+using namespace nans;
+
+extern "C" {
+
+const char *nans_last_error(void) { return g_error; }
+
+void nans_default_params(nans_params *p)
+{
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->solver_iterations = 70;
+    p->beta = 0.3f;
+    p->restitution = (float)0.1;
+    p->friction = 0.1f;
+    p->damping = 1.5f;
+    p->gravity = 9.81f;
+    p->dt = 1.0f / 60.0f;
+    p->max_gjk_iterations = 64;
+    p->max_epa_iterations = 64;
+    p->epa_tolerance = 0.001f;
+    p->fp64_islands = 0;
+    p->floor_pos[0] = 1.2f;
+    p->floor_pos[1] = -0.5f;
+    p->floor_pos[2] = 1.0f;
+    p->floor_scale[0] = 100.0f;
+    p->floor_scale[1] = 1.0f;
+    p->floor_scale[2] = 100.0f;
+    p->floor_mass = 100000.0f;
+    p->floor_moi = (p->floor_mass / 12.0f) * (2.0f * 100.0f * 100.0f);
+    p->device = 0;
+}
+
+static int check_params(const nans_params *p)
+{
+    NANS_REQUIRE(p, NANS_ERR_INVALID, "params is NULL");
+    NANS_REQUIRE(p->solver_iterations >= 0 && p->solver_iterations <= 100000, NANS_ERR_INVALID,
+                 "solver_iterations out of range");
+    NANS_REQUIRE(p->max_gjk_iterations >= 0 && p->max_gjk_iterations <= 100000, NANS_ERR_INVALID,
+                 "max_gjk_iterations out of range");
+    NANS_REQUIRE(p->max_epa_iterations >= 0 && p->max_epa_iterations <= 100000, NANS_ERR_INVALID,
+                 "max_epa_iterations out of range");
+    NANS_REQUIRE(p->fp64_islands == 0 || p->fp64_islands == 1, NANS_ERR_INVALID, "fp64_islands must be 0 or 1");
+    NANS_REQUIRE(p->floor_mass > 0.0f && p->floor_moi > 0.0f, NANS_ERR_INVALID, "floor mass/moi must be positive");
+    return NANS_OK;
+}
+
+int nans_create(nans_ctx **out, const nans_params *p)
+{
+    NANS_REQUIRE(out, NANS_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    nans_params q;
+    if (p) q = *p;
+    else nans_default_params(&q);
+    int rc = check_params(&q);
+    if (rc) return rc;
+    int ndev = 0;
+    NANS_HIP(hipGetDeviceCount(&ndev));
+    NANS_REQUIRE(q.device >= 0 && q.device < ndev, NANS_ERR_INVALID, "device %d not present (%d devices)", q.device, ndev);
+    NANS_HIP(hipSetDevice(q.device));
+    nans_ctx *c = new (std::nothrow) nans_ctx();
+    NANS_REQUIRE(c, NANS_ERR_CAPACITY, "out of host memory");
+    c->p = q;
+    c->stream = nullptr;
+    c->worlds = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        set_error("hipStreamCreateWithFlags: %s", hipGetErrorString(e));
+        return NANS_ERR_HIP;
+    }
+    e = hipMalloc((void **)&c->d_counters, 64 * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        hipStreamDestroy(c->own_stream);
+        delete c;
+        set_error("hipMalloc: %s", hipGetErrorString(e));
+        return NANS_ERR_HIP;
+    }
+    *out = c;
+    return NANS_OK;
+}
+
+int nans_destroy(nans_ctx *ctx)
+{
+    if (!ctx) return NANS_OK;
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    NANS_HIP(hipDeviceSynchronize());
+    worlds_release(ctx);
+    hipFree(ctx->d_counters);
+    hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return NANS_OK;
+}
+
+int nans_set_params(nans_ctx *ctx, const nans_params *p)
+{
+    NANS_REQUIRE(ctx, NANS_ERR_INVALID, "ctx is NULL");
+    int rc = check_params(p);
+    if (rc) return rc;
+    NANS_REQUIRE(p->device == ctx->p.device, NANS_ERR_INVALID, "the device of a context cannot change");
+    ctx->p = *p;
+    return NANS_OK;
+}
+
+int nans_set_stream(nans_ctx *ctx, void *hip_stream)
+{
+    NANS_REQUIRE(ctx, NANS_ERR_INVALID, "ctx is NULL");
+    ctx->stream = (hipStream_t)hip_stream;
+    return NANS_OK;
+}
+
+int nans_narrowphase(nans_ctx *ctx, int64_t n, const nans_shapes_soa *a, const nans_shapes_soa *b,
+                     const nans_narrow_out *out, int64_t *n_overflow)
+{
+    NANS_REQUIRE(ctx, NANS_ERR_INVALID, "ctx is NULL");
+    NANS_REQUIRE(n >= 0, NANS_ERR_INVALID, "n < 0");
+    NANS_REQUIRE(n <= (int64_t)1 << 31, NANS_ERR_CAPACITY, "n = %lld exceeds 2^31 pairs", (long long)n);
+    NANS_REQUIRE(a && b && out, NANS_ERR_INVALID, "shape or output descriptor is NULL");
+    NANS_REQUIRE(n == 0 || (a->type && a->pos && a->quat && a->half && b->type && b->pos && b->quat && b->half),
+                 NANS_ERR_INVALID, "a shape array is NULL");
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    hipStream_t s = ctx->stream;
+    k_fill_u64<<<1, 64, 0, s>>>(ctx->d_counters, 1, 0ull);
+    NANS_HIP(hipGetLastError());
+    if (n > 0) {
+        NarrowIn ia{a->type, a->pos, a->quat, a->half};
+        NarrowIn ib{b->type, b->pos, b->quat, b->half};
+        NarrowOutDev o{out->gjk, out->contact, out->status, out->normal, out->point_a, out->point_b, out->depth};
+        unsigned grid = (unsigned)((n + 255) / 256);
+        k_narrowphase<<<grid, 256, 0, s>>>(n, ia, ib, narrow_params(ctx->p), o, ctx->d_counters);
+        NANS_HIP(hipGetLastError());
+    }
+    unsigned long long ov = 0;
+    NANS_HIP(hipMemcpyAsync(&ov, ctx->d_counters, sizeof(ov), hipMemcpyDeviceToHost, s));
+    NANS_HIP(hipStreamSynchronize(s));
+    if (n_overflow) *n_overflow = (int64_t)ov;
+    return NANS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+"""Host-side binding of the C ABI (include/nans_gpu.h) over ctypes.
+
+This is the Python mirror of the reference's plugin boundary: the platform
+layer of the reference binds one C symbol with dlsym (code/sdl_nans.cpp:399-407)
+and calls it once per frame; here a Context binds libnans_gpu.so and exposes
+the same physics block (code/nans.cpp:1758-1762) as methods.  Device memory is
+handed over as torch tensors (PyTorch is only the allocator and stream here).
+The library is never replaced by a CPU path: if libnans_gpu.so is missing or
+no GPU is present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import scenes as _scenes
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnans_gpu.so")
+
+OK, ERR_INVALID, ERR_HIP, ERR_CAPACITY, ERR_STATE = 0, -1, -2, -3, -4
+SHAPE_BOX, SHAPE_SPHERE = 0, 1
+CC, CS, CF, SS, SF = 0, 1, 2, 3, 4
+GJK_NO, GJK_FOUND, GJK_CAP = 0, 1, 2
+EPA_NONE, EPA_CONTACT, EPA_BARY, EPA_NONFINITE, EPA_ITER, EPA_EMPTY, EPA_OVERFLOW = range(7)
+
+
+class NansError(RuntimeError):
+    """A C-ABI call returned an error code."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__("nans error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("solver_iterations", ctypes.c_int32), ("beta", ctypes.c_float), ("restitution", ctypes.c_float),
+                ("friction", ctypes.c_float), ("damping", ctypes.c_float), ("gravity", ctypes.c_float),
+                ("dt", ctypes.c_float), ("max_gjk_iterations", ctypes.c_int32), ("max_epa_iterations", ctypes.c_int32),
+                ("epa_tolerance", ctypes.c_float), ("fp64_islands", ctypes.c_int32),
+                ("floor_pos", ctypes.c_float * 3), ("floor_scale", ctypes.c_float * 3),
+                ("floor_mass", ctypes.c_float), ("floor_moi", ctypes.c_float), ("device", ctypes.c_int32)]
+
+
+class ShapesSoa(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_void_p), ("pos", ctypes.c_void_p), ("quat", ctypes.c_void_p),
+                ("half", ctypes.c_void_p)]
+
+
+class NarrowOut(ctypes.Structure):
+    _fields_ = [("gjk", ctypes.c_void_p), ("contact", ctypes.c_void_p), ("normal", ctypes.c_void_p),
+                ("depth", ctypes.c_void_p), ("point_a", ctypes.c_void_p), ("point_b", ctypes.c_void_p),
+                ("status", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libnans_gpu.so (built in-tree by nans_projekat_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError("%s is missing: run `python -m nans_projekat_amd.build`" % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.nans_last_error.restype = ctypes.c_char_p
+        _lib = lib
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        raise NansError(rc, load().nans_last_error().decode(errors="replace"))
+
+
+def to_params(sp: "_scenes.StepParams | None" = None, device: int = 0, **overrides) -> Params:
+    """nans_params from a StepParams (reference defaults when None)."""
+    p = Params()
+    load().nans_default_params(ctypes.byref(p))
+    if sp is not None:
+        for name in ("solver_iterations", "beta", "restitution", "friction", "damping", "gravity", "dt",
+                     "max_gjk_iterations", "max_epa_iterations", "epa_tolerance", "fp64_islands",
+                     "floor_mass", "floor_moi"):
+            setattr(p, name, getattr(sp, name))
+        p.floor_pos = (ctypes.c_float * 3)(*sp.floor_pos)
+        p.floor_scale = (ctypes.c_float * 3)(*sp.floor_scale)
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    p.device = device
+    return p
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+class Context:
+    """Owns a nans_ctx.  Calls are ordered on torch's current stream of the
+    context's device, so tensors produced by torch are ready when a call runs."""
+
+    def __init__(self, params: "_scenes.StepParams | None" = None, device: int = 0, **overrides):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("no GPU: the HIP step has no CPU fallback")
+        self.torch = torch
+        self.device = device
+        self.lib = load()
+        self.params = to_params(params, device, **overrides)
+        h = ctypes.c_void_p()
+        check(self.lib.nans_create(ctypes.byref(h), ctypes.byref(self.params)))
+        self.h = h
+        self._sync_stream()
+
+    def _sync_stream(self):
+        s = self.torch.cuda.current_stream(self.device).cuda_stream
+        check(self.lib.nans_set_stream(self.h, ctypes.c_void_p(s)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(self.lib.nans_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params: "_scenes.StepParams | None" = None, **overrides):
+        self.params = to_params(params, self.device, **overrides)
+        check(self.lib.nans_set_params(self.h, ctypes.byref(self.params)))
+
+    # ------------------------------------------------------------------ narrowphase
+    def upload_pairs(self, pairs: dict) -> dict:
+        """Host pairs ([n][3]/[n][4] numpy, scenes.config2_pairs layout) -> device
+        SoA planes ([3][n]/[4][n] torch tensors)."""
+        torch = self.torch
+        dev = "cuda:%d" % self.device
+        out = {}
+        for side in ("a", "b"):
+            out["type_" + side] = torch.from_numpy(np.ascontiguousarray(pairs["type_" + side], np.int32)).to(dev)
+            for k in ("pos", "quat", "half"):
+                arr = np.ascontiguousarray(np.asarray(pairs["%s_%s" % (k, side)], np.float32).T)
+                out["%s_%s" % (k, side)] = torch.from_numpy(arr).to(dev)
+        out["n"] = len(pairs["type_a"])
+        return out
+
+    def alloc_narrow_out(self, n: int) -> dict:
+        torch = self.torch
+        dev = "cuda:%d" % self.device
+        return dict(gjk=torch.empty(n, dtype=torch.int32, device=dev),
+                    contact=torch.empty(n, dtype=torch.int32, device=dev),
+                    normal=torch.empty((3, n), dtype=torch.float32, device=dev),
+                    depth=torch.empty(n, dtype=torch.float64, device=dev),
+                    point_a=torch.empty((3, n), dtype=torch.float32, device=dev),
+                    point_b=torch.empty((3, n), dtype=torch.float32, device=dev),
+                    status=torch.empty(n, dtype=torch.int32, device=dev))
+
+    def narrowphase(self, dpairs: dict, out: dict | None = None) -> dict:
+        """CheckCollision (code/nans.cpp:906-966) on device-resident pairs."""
+        n = int(dpairs["n"])
+        if out is None:
+            out = self.alloc_narrow_out(n)
+        a = ShapesSoa(*[_ptr(dpairs[k + "_a"]) for k in ("type", "pos", "quat", "half")])
+        b = ShapesSoa(*[_ptr(dpairs[k + "_b"]) for k in ("type", "pos", "quat", "half")])
+        o = NarrowOut(*[_ptr(out[k]) for k in ("gjk", "contact", "normal", "depth", "point_a", "point_b", "status")])
+        nov = ctypes.c_int64(0)
+        self._sync_stream()
+        check(self.lib.nans_narrowphase(self.h, ctypes.c_int64(n), ctypes.byref(a), ctypes.byref(b),
+                                        ctypes.byref(o), ctypes.byref(nov)))
+        out["n_overflow"] = nov.value
+        return out
+
+    @staticmethod
+    def narrow_to_numpy(out: dict) -> dict:
+        """Device outputs -> host arrays in the [n][3] layout of the oracle."""
+        res = {}
+        for k, v in out.items():
+            if k == "n_overflow":
+                res[k] = v
+                continue
+            a = v.cpu().numpy()
+            res[k] = np.ascontiguousarray(a.T) if a.ndim == 2 else a
+        return res

@@ -1,0 +1,182 @@
+/*
+ * nans_oracle.h -- CPU oracle for the nans_projekat simulation step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a literal C restatement of the
+ * reference's physics block (code/nans.cpp) used as the checker for the
+ * HIP implementation.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it; the product library never
+ * links, loads or calls it.
+ *
+ * The same source (nans_oracle_impl.h) is compiled twice:
+ *   *_f32 : binary32 arithmetic with the reference's double islands
+ *           (code/nans.cpp:68 (real32)(1.0/Mass), :1199/:1214 sqrt(2)*Cf*Sum),
+ *           built -O2 -ffp-contract=off -fno-fast-math on x86-64 SSE, which
+ *           reproduces the reference's g++ -O0 SSE scalar arithmetic
+ *           (code/build.sh:5-14: no -O flag, no FMA).
+ *   *_f64 : the same algorithm in binary64 (the "FP64 oracle").
+ *
+ * Vector arithmetic follows glm's published source semantics (glm is an
+ * un-vendored dependency of the reference, code/nans.h:6-10, version not
+ * pinned; the forms used are those of glm 0.9.8/0.9.9):
+ *   dot(a,b)      = (a.x*b.x + a.y*b.y) + a.z*b.z
+ *   cross(a,b)    = (a.y*b.z - b.y*a.z, a.z*b.x - b.z*a.x, a.x*b.y - b.x*a.y)
+ *   normalize(v)  = v * (1 / sqrt(dot(v,v)))
+ *   length(v)     = sqrt(dot(v,v))
+ *   abs(x)        = x >= 0 ? x : -x
+ *   mat4 * vec4   = (c0*x + c1*y) + (c2*z + c3*w)
+ *   translate, rotate (axis-angle), scale, radians as in matrix_transform.inl.
+ */
+#ifndef NANS_ORACLE_H
+#define NANS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Shape kinds of the narrowphase entry point. */
+#define ORA_SHAPE_BOX 0
+#define ORA_SHAPE_SPHERE 1
+
+/* Pair kinds, numbered as collision_type in code/nans.h:73-89. */
+#define ORA_CC 0
+#define ORA_CS 1
+#define ORA_CF 2
+#define ORA_SS 3
+#define ORA_SF 4
+
+/* GJK outcome (evolve_result, code/nans.h:91-96) plus the iteration cap. */
+#define ORA_GJK_NO 0
+#define ORA_GJK_FOUND 1
+#define ORA_GJK_CAP 2
+
+/* EPA outcome codes. */
+#define ORA_EPA_NONE 0        /* EPA not run (GJK did not find an intersection) */
+#define ORA_EPA_CONTACT 1     /* converged, contact written (nans.cpp:828-866) */
+#define ORA_EPA_BARY 2        /* |barycentric| > 1 -> no contact (nans.cpp:836-840) */
+#define ORA_EPA_NONFINITE 3   /* NaN/inf barycentric -> no contact (nans.cpp:842-846) */
+#define ORA_EPA_ITER 4        /* 65 iterations without convergence (nans.cpp:903) */
+#define ORA_EPA_EMPTY 5       /* every face removed: the reference reads Triangle[0]
+                                 of an empty vector here (undefined behaviour); the
+                                 oracle reports no contact */
+
+/* Per-pair statistics written by the narrowphase entry points. */
+#define ORA_STAT_GJK_CALLS 0   /* EvolveSimplex calls */
+#define ORA_STAT_GJK_RESULT 1  /* ORA_GJK_* */
+#define ORA_STAT_EPA_ITERS 2   /* EPA loop iterations entered */
+#define ORA_STAT_EPA_RESULT 3  /* ORA_EPA_* */
+#define ORA_STAT_MAX_FACES 4   /* largest face count seen */
+#define ORA_STAT_MAX_EDGES 5   /* largest horizon edge list */
+#define ORA_STAT_NVERTS 6      /* polytope vertices at exit */
+#define ORA_STAT_FACES_MADE 7  /* faces created in total */
+#define ORA_NSTAT 8
+
+typedef struct ora_np_params {
+    int32_t max_gjk_iterations; /* MAX_GJK_ITERATIONS (64): the loop runs 65 calls */
+    int32_t max_epa_iterations; /* MAX_EPA_ITERATIONS (64): the loop runs 65 times */
+    float epa_tolerance;        /* MAX_EPA_ERROR (0.001f) */
+    int32_t fp64_islands;       /* 0: reference arithmetic; 1: GJK length test and
+                                   EPA final face choice / depth in binary64 */
+} ora_np_params;
+
+typedef struct ora_step_params {
+    ora_np_params np;
+    int32_t solver_iterations;  /* 70 in the reference (nans.cpp:1176) */
+    float beta;                 /* 0.3f (nans.cpp:1152) */
+    float restitution;          /* Cr, 0.1 (nans.cpp:1154) */
+    float friction;             /* Cf, 0.1f (nans.cpp:1159) */
+    float damping;              /* GLOBAL_FRICTION 1.5f (nans.h:46) */
+    float gravity;              /* 9.81f (nans.cpp:68) */
+    float dt;
+    float floor_pos[3];         /* (1.2, -0.5, 1.0) (nans.cpp:1669) */
+    float floor_scale[3];       /* (Size, 1, Size) = (100, 1, 100) (nans.cpp:1878) */
+    float floor_mass;           /* 1e5 (nans.cpp:1676) */
+    float floor_moi;            /* (m/12)*(2*Size*Size) (nans.cpp:1677) */
+} ora_step_params;
+
+/* One contact of the ordered contact list (contact_pair, code/nans.h:339-372). */
+typedef struct ora_contact_f32 {
+    int32_t type, index_a, index_b, pad;
+    float point_a[3], point_b[3], normal[3];
+    float dl_normal, dl_tangent1, dl_tangent2; /* impulses applied by Constraint */
+} ora_contact_f32;
+
+typedef struct ora_contact_f64 {
+    int32_t type, index_a, index_b, pad;
+    double point_a[3], point_b[3], normal[3];
+    double dl_normal, dl_tangent1, dl_tangent2;
+} ora_contact_f64;
+
+/* Narrowphase over n explicit pairs.  Arrays are interleaved per pair:
+ * pos [n][3], quat [n][4] (w,x,y,z), half [n][3] (spheres: half[0] = radius).
+ * Box vertices are built as Model = [R(q)*2h | c] applied to the corners
+ * (+-0.5,+-0.5,+-0.5,1) in the order of code/nans.cpp:399-406.
+ * Outputs: gjk (ORA_GJK_*), contact (0/1), normal [n][3], depth (double),
+ * point_a/point_b [n][3]; stats [n][ORA_NSTAT] may be NULL.
+ * Returns 0, or -1 on bad arguments. */
+int ora_narrowphase_f32(int64_t n,
+                        const int32_t *type_a, const float *pos_a, const float *quat_a, const float *half_a,
+                        const int32_t *type_b, const float *pos_b, const float *quat_b, const float *half_b,
+                        const ora_np_params *p,
+                        int32_t *gjk, int32_t *contact, float *normal, double *depth,
+                        float *point_a, float *point_b, int32_t *stats);
+int ora_narrowphase_f64(int64_t n,
+                        const int32_t *type_a, const double *pos_a, const double *quat_a, const double *half_a,
+                        const int32_t *type_b, const double *pos_b, const double *quat_b, const double *half_b,
+                        const ora_np_params *p,
+                        int32_t *gjk, int32_t *contact, double *normal, double *depth,
+                        double *point_a, double *point_b, int32_t *stats);
+
+/* Advance one world by nsteps.  The world holds nb boxes followed by ns spheres;
+ * per-body arrays are interleaved [n][3] (n = nb + ns):
+ *   pos, vel, angvel, angles (degrees, as code/nans.cpp:1915-1920 reads them),
+ *   force, torque  (in/out; cleared by IntegrateForces, nans.cpp:92-110),
+ *   scale          (boxes: Model scale = full extents; spheres: scale[0] = radius),
+ *   mass, moi      [n].
+ * The box Model is rebuilt from the pose at entry (the initial Model is built from
+ * the initial pose) and after every step (nans.cpp:1913-1941).
+ * If contacts != NULL the contact list of the LAST step is written there (up to
+ * max_contacts entries) and its length to *ncontacts.  stats (may be NULL):
+ *   [0] pair tests, [1] GJK found, [2] contacts, [3] max EPA faces, [4] max EPA edges,
+ *   [5] max EPA vertices, [6] EPA iteration-cap exits, [7] EPA empty-polytope exits
+ * accumulated over all steps.  Returns 0, or -1 on bad arguments. */
+int ora_world_run_f32(int32_t nb, int32_t ns,
+                      float *pos, float *vel, float *angvel, float *angles,
+                      float *force, float *torque,
+                      const float *scale, const float *mass, const float *moi,
+                      const ora_step_params *p, int32_t nsteps,
+                      ora_contact_f32 *contacts, int32_t max_contacts, int32_t *ncontacts,
+                      int64_t *stats);
+int ora_world_run_f64(int32_t nb, int32_t ns,
+                      double *pos, double *vel, double *angvel, double *angles,
+                      double *force, double *torque,
+                      const double *scale, const double *mass, const double *moi,
+                      const ora_step_params *p, int32_t nsteps,
+                      ora_contact_f64 *contacts, int32_t max_contacts, int32_t *ncontacts,
+                      int64_t *stats);
+
+/* Every canonical pair of the world's current pose (DetectCollisions order,
+ * nans.cpp:1355-1535): type/index_a/index_b and the per-pair stats of
+ * CheckCollision, without solving or stepping.  Arrays have room for
+ * npairs = nb(nb-1)/2 + nb + ns + nb*ns + ns(ns-1)/2 entries. */
+int ora_world_pairs_f32(int32_t nb, int32_t ns,
+                        const float *pos, const float *angles, const float *scale,
+                        const ora_step_params *p,
+                        int32_t *type, int32_t *index_a, int32_t *index_b, int32_t *stats);
+
+/* Box corners of the Model built from a pose (translate * rotX * rotY * rotZ * scale,
+ * degrees), in the order of code/nans.cpp:399-406: out [8][3]. */
+void ora_box_vertices_f32(const float *pos, const float *angles, const float *scale, float *out);
+void ora_box_vertices_f64(const double *pos, const double *angles, const double *scale, double *out);
+
+/* RK4 of code/nans.cpp:50-78 on one body: v, w in/out [3]. */
+void ora_rk4_f32(float *v, float *w, const float *force, const float *torque,
+                 float mass, float moi, const ora_step_params *p);
+void ora_rk4_f64(double *v, double *w, const double *force, const double *torque,
+                 double mass, double moi, const ora_step_params *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

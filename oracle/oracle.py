@@ -1,0 +1,193 @@
+"""ctypes front end of the CPU oracle (oracle/nans_oracle_impl.h).
+
+TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker of the HIP path.  The product
+package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libnans_oracle.so")
+LIB_O0_PATH = os.path.join(HERE, "lib", "libnans_oracle_O0.so")
+
+NSTAT = 8
+STAT_GJK_CALLS, STAT_GJK_RESULT, STAT_EPA_ITERS, STAT_EPA_RESULT = 0, 1, 2, 3
+STAT_MAX_FACES, STAT_MAX_EDGES, STAT_NVERTS, STAT_FACES_MADE = 4, 5, 6, 7
+GJK_NO, GJK_FOUND, GJK_CAP = 0, 1, 2
+EPA_NONE, EPA_CONTACT, EPA_BARY, EPA_NONFINITE, EPA_ITER, EPA_EMPTY = 0, 1, 2, 3, 4, 5
+CC, CS, CF, SS, SF = 0, 1, 2, 3, 4
+
+
+class NpParams(ctypes.Structure):
+    _fields_ = [("max_gjk_iterations", ctypes.c_int32), ("max_epa_iterations", ctypes.c_int32),
+                ("epa_tolerance", ctypes.c_float), ("fp64_islands", ctypes.c_int32)]
+
+
+class StepParams(ctypes.Structure):
+    _fields_ = [("np", NpParams), ("solver_iterations", ctypes.c_int32), ("beta", ctypes.c_float),
+                ("restitution", ctypes.c_float), ("friction", ctypes.c_float), ("damping", ctypes.c_float),
+                ("gravity", ctypes.c_float), ("dt", ctypes.c_float), ("floor_pos", ctypes.c_float * 3),
+                ("floor_scale", ctypes.c_float * 3), ("floor_mass", ctypes.c_float), ("floor_moi", ctypes.c_float)]
+
+
+class Contact32(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("index_a", ctypes.c_int32), ("index_b", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("point_a", ctypes.c_float * 3), ("point_b", ctypes.c_float * 3),
+                ("normal", ctypes.c_float * 3), ("dl_normal", ctypes.c_float), ("dl_tangent1", ctypes.c_float),
+                ("dl_tangent2", ctypes.c_float)]
+
+
+class Contact64(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("index_a", ctypes.c_int32), ("index_b", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("point_a", ctypes.c_double * 3), ("point_b", ctypes.c_double * 3),
+                ("normal", ctypes.c_double * 3), ("dl_normal", ctypes.c_double), ("dl_tangent1", ctypes.c_double),
+                ("dl_tangent2", ctypes.c_double)]
+
+
+CONTACT_DTYPE = {
+    "f32": np.dtype([("type", "<i4"), ("index_a", "<i4"), ("index_b", "<i4"), ("pad", "<i4"),
+                     ("point_a", "<f4", 3), ("point_b", "<f4", 3), ("normal", "<f4", 3),
+                     ("dl_normal", "<f4"), ("dl_tangent1", "<f4"), ("dl_tangent2", "<f4")]),
+    "f64": np.dtype([("type", "<i4"), ("index_a", "<i4"), ("index_b", "<i4"), ("pad", "<i4"),
+                     ("point_a", "<f8", 3), ("point_b", "<f8", 3), ("normal", "<f8", 3),
+                     ("dl_normal", "<f8"), ("dl_tangent1", "<f8"), ("dl_tangent2", "<f8")]),
+}
+
+_libs: dict = {}
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (gcc only)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib(opt: str = "O2"):
+    path = LIB_PATH if opt == "O2" else LIB_O0_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            build()
+        _libs[path] = ctypes.CDLL(path)
+    return _libs[path]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def np_params(params) -> NpParams:
+    return NpParams(int(params.max_gjk_iterations), int(params.max_epa_iterations),
+                    float(params.epa_tolerance), int(params.fp64_islands))
+
+
+def step_params(params, friction=None, restitution=None) -> StepParams:
+    p = StepParams()
+    p.np = np_params(params)
+    p.solver_iterations = int(params.solver_iterations)
+    p.beta = params.beta
+    p.restitution = params.restitution if restitution is None else float(restitution)
+    p.friction = params.friction if friction is None else float(friction)
+    p.damping = params.damping
+    p.gravity = params.gravity
+    p.dt = params.dt
+    p.floor_pos = (ctypes.c_float * 3)(*params.floor_pos)
+    p.floor_scale = (ctypes.c_float * 3)(*params.floor_scale)
+    p.floor_mass = params.floor_mass
+    p.floor_moi = params.floor_moi
+    return p
+
+
+def narrowphase(pairs: dict, params, precision: str = "f32", stats: bool = False, opt: str = "O2"):
+    """CheckCollision on explicit pairs (code/nans.cpp:906-966)."""
+    ft = np.float32 if precision == "f32" else np.float64
+    n = len(pairs["type_a"])
+    c = lambda k: np.ascontiguousarray(pairs[k], dtype=ft)
+    i32 = lambda k: np.ascontiguousarray(pairs[k], dtype=np.int32)
+    out = dict(gjk=np.zeros(n, np.int32), contact=np.zeros(n, np.int32), normal=np.zeros((n, 3), ft),
+               depth=np.zeros(n, np.float64), point_a=np.zeros((n, 3), ft), point_b=np.zeros((n, 3), ft))
+    st = np.zeros((n, NSTAT), np.int32) if stats else None
+    args = [i32("type_a"), c("pos_a"), c("quat_a"), c("half_a"), i32("type_b"), c("pos_b"), c("quat_b"), c("half_b")]
+    fn = getattr(lib(opt), "ora_narrowphase_" + precision)
+    prm = np_params(params)
+    rc = fn(ctypes.c_int64(n), *[_ptr(a) for a in args], ctypes.byref(prm),
+            _ptr(out["gjk"]), _ptr(out["contact"]), _ptr(out["normal"]), _ptr(out["depth"]),
+            _ptr(out["point_a"]), _ptr(out["point_b"]), _ptr(st))
+    if rc != 0:
+        raise ValueError("oracle narrowphase rejected its arguments")
+    if stats:
+        out["stats"] = st
+    return out
+
+
+class WorldState:
+    """One world's mutable state for the oracle (interleaved [n][3] arrays)."""
+
+    def __init__(self, scene, w: int = 0, precision: str = "f32"):
+        ft = np.float32 if precision == "f32" else np.float64
+        a, b = int(scene.offsets[w]), int(scene.offsets[w + 1])
+        self.precision = precision
+        self.nb = int(scene.nboxes[w])
+        self.ns = (b - a) - self.nb
+        cp = lambda x: np.ascontiguousarray(x[a:b], dtype=ft)
+        self.pos, self.vel, self.angvel, self.angles = cp(scene.pos), cp(scene.vel), cp(scene.angvel), cp(scene.angles)
+        self.force, self.torque = cp(scene.force), cp(scene.torque)
+        self.scale, self.mass, self.moi = cp(scene.scale), cp(scene.mass), cp(scene.moi)
+        self.params = scene.params
+        self.friction = None if scene.world_friction is None else float(scene.world_friction[w])
+        self.restitution = None if scene.world_restitution is None else float(scene.world_restitution[w])
+        self.stats = np.zeros(8, np.int64)
+
+    def run(self, nsteps: int, contacts: bool = False, max_contacts: int = 1 << 16, opt: str = "O2"):
+        """Advance nsteps; returns the contact list of the last step if asked."""
+        prm = step_params(self.params, self.friction, self.restitution)
+        fn = getattr(lib(opt), "ora_world_run_" + self.precision)
+        cbuf = np.zeros(max_contacts, CONTACT_DTYPE[self.precision]) if contacts else None
+        ncon = ctypes.c_int32(0)
+        rc = fn(self.nb, self.ns, _ptr(self.pos), _ptr(self.vel), _ptr(self.angvel), _ptr(self.angles),
+                _ptr(self.force), _ptr(self.torque), _ptr(self.scale), _ptr(self.mass), _ptr(self.moi),
+                ctypes.byref(prm), int(nsteps), _ptr(cbuf), int(max_contacts), ctypes.byref(ncon),
+                _ptr(self.stats))
+        if rc != 0:
+            raise ValueError("oracle world_run rejected its arguments")
+        if contacts:
+            return cbuf[: min(ncon.value, max_contacts)].copy()
+        return None
+
+    def pairs(self):
+        """Every canonical pair of the current pose with its CheckCollision stats."""
+        nb, ns = self.nb, self.ns
+        npairs = nb * (nb - 1) // 2 + nb + ns + nb * ns + ns * (ns - 1) // 2
+        t, ia, ib = (np.zeros(npairs, np.int32) for _ in range(3))
+        st = np.zeros((npairs, NSTAT), np.int32)
+        prm = step_params(self.params, self.friction, self.restitution)
+        f32 = lambda x: np.ascontiguousarray(x, dtype=np.float32)
+        rc = lib().ora_world_pairs_f32(nb, ns, _ptr(f32(self.pos)), _ptr(f32(self.angles)), _ptr(f32(self.scale)),
+                                       ctypes.byref(prm), _ptr(t), _ptr(ia), _ptr(ib), _ptr(st))
+        if rc != 0:
+            raise ValueError("oracle world_pairs rejected its arguments")
+        return t, ia, ib, st
+
+
+def box_vertices(pos, angles, scale, precision: str = "f32"):
+    ft = np.float32 if precision == "f32" else np.float64
+    out = np.zeros((8, 3), ft)
+    a = [np.ascontiguousarray(x, dtype=ft) for x in (pos, angles, scale)]
+    getattr(lib(), "ora_box_vertices_" + precision)(*[_ptr(x) for x in a], _ptr(out))
+    return out
+
+
+def rk4(v, w, force, torque, mass, moi, params, precision: str = "f32"):
+    ft = np.float32 if precision == "f32" else np.float64
+    cty = ctypes.c_float if precision == "f32" else ctypes.c_double
+    v = np.array(v, ft)
+    w = np.array(w, ft)
+    f = np.ascontiguousarray(force, dtype=ft)
+    t = np.ascontiguousarray(torque, dtype=ft)
+    prm = step_params(params)
+    getattr(lib(), "ora_rk4_" + precision)(_ptr(v), _ptr(w), _ptr(f), _ptr(t), cty(mass), cty(moi), ctypes.byref(prm))
+    return v, w
