@@ -134,6 +134,76 @@ int nans_set_stream(nans_ctx *ctx, void *hip_stream);
 int nans_narrowphase(nans_ctx *ctx, int64_t n, const nans_shapes_soa *a, const nans_shapes_soa *b,
                      const nans_narrow_out *out, int64_t *n_overflow);
 
+/* ------------------------------------------------------------------ worlds */
+
+/* Bodies of a batch of worlds, host pointers.  World w owns bodies
+ * [world_offsets[w], world_offsets[w+1]): its boxes first, then its spheres (the
+ * reference's State->Cubes[] and State->Spheres[], code/nans.h:377-378; pair
+ * indices are relative to those two spaces).  Each world has its own copy of the
+ * static floor box of nans_params.  Vector arrays are [3][N] planes. */
+typedef struct nans_bodies_soa {
+    float *pos;            /* Position */
+    float *vel;            /* V */
+    float *angvel;         /* W */
+    float *angles;         /* Angles, in degrees as the Model rebuild reads them */
+    float *force;          /* Forces (may be NULL: zero); cleared by each step */
+    float *torque;         /* Torque (may be NULL: zero); cleared by each step */
+    const float *scale;    /* boxes: Model scale (full extents); spheres: plane 0 = Radius */
+    const float *mass;     /* [N] */
+    const float *moi;      /* [N] scalar moment of inertia */
+    const int32_t *kind;   /* [N] NANS_SHAPE_* */
+} nans_bodies_soa;
+
+/* One contact of a world's ordered contact list (contact_pair, code/nans.h:339-372):
+ * the pair as DetectCollisions lists it and the impulses Constraint applied. */
+typedef struct nans_contact {
+    int32_t type;          /* NANS_CC ... NANS_SF */
+    int32_t index_a, index_b;  /* Cubes[]/Spheres[] indices; floor pairs repeat index_a */
+    int32_t pad;
+    float point_a[3], point_b[3], normal[3];
+    float dl_normal, dl_tangent1, dl_tangent2;
+} nans_contact;
+
+/* Counters accumulated since the upload or nans_reset_stats. */
+typedef struct nans_stats {
+    int64_t pair_tests;        /* canonical pairs considered */
+    int64_t candidates;        /* pairs that reached GJK */
+    int64_t gjk_found;         /* GJK intersections */
+    int64_t contacts;          /* EPA contacts */
+    int64_t epa_overflow;      /* NANS_EPA_OVERFLOW pairs */
+    int64_t capacity_overflow; /* steps whose candidates or contacts exceeded scratch */
+} nans_stats;
+
+/* Copy a batch of worlds to the device (replacing any previous batch).  Per-world
+ * friction / restitution arrays are optional (NULL: the params' values); the
+ * reference has one global Cf and Cr (code/nans.cpp:1154,1159).  At most 16384
+ * bodies per world. */
+int nans_upload_worlds(nans_ctx *ctx, int32_t nworlds, const int32_t *world_offsets, const nans_bodies_soa *bodies,
+                       const float *world_friction, const float *world_restitution);
+
+/* Copy pos, vel, angvel, angles, force, torque of every body back (non-NULL fields). */
+int nans_download_state(nans_ctx *ctx, nans_bodies_soa *bodies);
+
+/* Advance every world by nsteps with the plain path: one world per launch
+ * sequence (the drop-in for the reference's physics block). */
+int nans_step_plain(nans_ctx *ctx, int32_t nsteps);
+
+/* Advance every world by nsteps with the batched path: one persistent launch per
+ * step over all worlds (at most 512 bodies per world), the steps of the call
+ * captured in a hipGraph that later calls with the same nsteps replay. */
+int nans_step_batched(nans_ctx *ctx, int32_t nsteps);
+
+/* Keep the contact list of each world's last step (off by default); at most
+ * 16 n + 64 contacts per world of n bodies are kept. */
+int nans_record_contacts(nans_ctx *ctx, int32_t enable);
+
+/* The recorded contact list of a world: *n receives its length, up to
+ * max_contacts entries are copied to out (host memory). */
+int nans_get_contacts(nans_ctx *ctx, int32_t world, nans_contact *out, int32_t max_contacts, int32_t *n);
+
+int nans_get_stats(nans_ctx *ctx, nans_stats *stats);
+int nans_reset_stats(nans_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,17 +1,1140 @@
-// nans_step.hip -- world batches: upload/download and the simulation step.
+// nans_step.hip -- world batches and the simulation step.
+//
+// One step is the reference's physics block (code/nans.cpp:1758-1762):
+//   IntegrateForces -> DetectCollisions -> SolveConstraints -> IntegrateVelocities
+// followed by the Model rebuild of its render section (code/nans.cpp:1870-1881,
+// 1913-1941), whose box corners the next step's narrowphase reads.  Here the
+// Model and corners are rebuilt from the pose at the start of each step, which is
+// the same value.
+//
+// Two launch structures share the device routines of nans_world.hpp:
+//   plain    one world per launch sequence, grid-wide kernels:
+//            forces+corners | cull count | scan | cull write | narrowphase | solve
+//   batched  one persistent launch per step; each workgroup takes worlds from a
+//            queue (largest first) and runs the whole step of a world with its
+//            bodies in LDS; the steps of a call are captured in a hipGraph.
+//
+// SolveConstraints applies the contacts one after another in list order
+// (code/nans.cpp:1541-1547), each reading the velocities left by the previous
+// ones.  Both paths reproduce that order exactly with dependency rounds: in each
+// round a contact is applied iff it is the earliest pending contact of each of its
+// dynamic bodies.  Contacts applied in one round touch disjoint bodies, and every
+// body sees its contacts in list order, so the result equals the serial loop bit
+// for bit.  The floor's velocities are never written, so it orders nothing.
 #include "nans_common.hpp"
+#include "nans_world.hpp"
 
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <map>
+#include <numeric>
+#include <vector>
+
+using namespace nans;
+
+namespace nans {
+
+constexpr int PLAIN_MAX_BODIES = 16384;  // solver head[] lives in LDS (64 KiB)
+constexpr int BATCH_MAX_BODIES = 512;    // bodies of one world kept in LDS by the batched path
+constexpr int SCAN_THREADS = 1024;
+constexpr int SOLVE_THREADS = 1024;
+constexpr int CULL_CHUNK = 4096;          // canonical pairs per block of the plain cull
+
+enum : int {
+    CNT_EPA_OVERFLOW = 0,
+    CNT_CAND_OVERFLOW,
+    CNT_CON_OVERFLOW,
+    CNT_CANDIDATES,
+    CNT_CONTACTS,
+    CNT_GJK_FOUND,
+    CNT_PAIR_TESTS,
+    CNT_N
+};
+
+// ------------------------------------------------------------------ small block helpers
+template <int BLOCK>
+__device__ inline int block_rank(bool pred, int *wave_tot, int &total)
+{
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    uint64_t m = ballot(pred);
+    int r = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_tot[w] = __popcll(m);
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < BLOCK / 64; ++i) {
+        int t = wave_tot[i];
+        off += i < w ? t : 0;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return off + r;
+}
+
+__device__ inline Shape body_shape(const Planes &B, const float *verts, int slot, int nb, const FloorDev &fl)
+{
+    Shape s;
+    if (slot < 0) {
+        s.kind = SHAPE_BOX;
+        s.c = mk3(fl.c[0], fl.c[1], fl.c[2]);
+        s.r = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s.v[k] = mk3(fl.v[3 * k], fl.v[3 * k + 1], fl.v[3 * k + 2]);
+    } else if (slot < nb) {
+        s.kind = SHAPE_BOX;
+        s.c = B.get3(F_PX, slot);
+        s.r = 0.0f;
+        const float *v = verts + 24 * slot;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s.v[k] = mk3(v[3 * k], v[3 * k + 1], v[3 * k + 2]);
+    } else {
+        s.kind = SHAPE_SPHERE;
+        s.c = B.get3(F_PX, slot);
+        s.r = B.at(F_SX, slot);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s.v[k] = mk3(0.0f, 0.0f, 0.0f);
+    }
+    return s;
+}
+
+__device__ inline void slot_sphere(const Planes &B, const float *brad, int slot, const FloorDev &fl, V3 &c, float &r)
+{
+    if (slot < 0) {
+        c = mk3(fl.c[0], fl.c[1], fl.c[2]);
+        r = fl.brad;
+    } else {
+        c = B.get3(F_PX, slot);
+        r = brad[slot];
+    }
+}
+
+// GJK on one lane per candidate, EPA by the wave for each GJK hit; fills rec and
+// returns whether the candidate produced a contact.  All 64 lanes must call it.
+__device__ inline bool narrow_candidate(bool active, int64_t pair, int nb, int ns, const Planes &B,
+                                        const float *verts, const FloorDev &fl, const NarrowParams &np,
+                                        EpaLds &L, ContactRec &rec, unsigned long long *cnt)
+{
+    const int lane = lane_id();
+    Shape A, Bs;
+    Simplex S;
+    S.n = 0;
+    int g = GJK_NO, calls = 0;
+    PairId q{0, 0, 0};
+    int sa = 0, sb = 0;
+    if (active) {
+        q = decode_pair(pair, nb, ns);
+        pair_slots(q, nb, sa, sb);
+        A = body_shape(B, verts, sa, nb, fl);
+        Bs = body_shape(B, verts, sb, nb, fl);
+        g = gjk(A, Bs, np, S, &calls);
+    }
+    int status = EPA_NONE;
+    V3 pa = mk3(0.0f, 0.0f, 0.0f), pb = pa, nrm = pa;
+    uint64_t m = ballot(active && g == GJK_FOUND);
+    if (lane == 0 && m) atomicAdd(&cnt[CNT_GJK_FOUND], (unsigned long long)__popcll(m));
+    while (m) {
+        const int src = ffs64(m);
+        m &= m - 1;
+        if (lane == src) {
+            shape_store(L.sa, A);
+            shape_store(L.sb, Bs);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                st3(L.vp, k, S.s[k].P);
+                st3(L.va, k, S.s[k].A);
+                st3(L.vb, k, S.s[k].B);
+            }
+        }
+        wave_sync();
+        EpaResult r = epa_wave(L, np);
+        if (lane == src) {
+            status = r.status;
+            pa = r.pa;
+            pb = r.pb;
+            nrm = r.n;
+        }
+        wave_sync();
+    }
+    if (status == EPA_OVERFLOW) atomicAdd(&cnt[CNT_EPA_OVERFLOW], 1ull);
+    rec.type = q.type;
+    rec.ia = q.i;
+    rec.ib = q.j;
+    rec.slot_a = sa;
+    rec.slot_b = sb;
+    rec.pad0 = rec.pad1 = rec.pad2 = 0;
+    rec.pa[0] = pa.x; rec.pa[1] = pa.y; rec.pa[2] = pa.z;
+    rec.pb[0] = pb.x; rec.pb[1] = pb.y; rec.pb[2] = pb.z;
+    rec.n[0] = nrm.x; rec.n[1] = nrm.y; rec.n[2] = nrm.z;
+    rec.dln = rec.dlt1 = rec.dlt2 = 0.0f;
+    return active && status == EPA_CONTACT;
+}
+
+// Dependency rounds of SolveConstraints over con[0..ncon) (see the file comment).
+// head[] has room for n ints; done[] for ncon.
+template <int BLOCK>
+__device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int ncon, int *done, int *head,
+                                    const FloorDev &fl, const StepConsts &k, float Cf, float Cr)
+{
+    const int tid = threadIdx.x;
+    for (int c = tid; c < ncon; c += BLOCK) done[c] = 0;
+    __syncthreads();
+    int pending = ncon;
+    while (pending > 0) {
+        for (int i = tid; i < n; i += BLOCK) head[i] = INT_MAX;
+        __syncthreads();
+        for (int c = tid; c < ncon; c += BLOCK) {
+            if (!done[c]) {
+                atomicMin(&head[con[c].slot_a], c);
+                int sb = con[c].slot_b;
+                if (sb >= 0) atomicMin(&head[sb], c);
+            }
+        }
+        __syncthreads();
+        int left = 0;
+        for (int c = tid; c < ncon; c += BLOCK) {
+            if (!done[c]) {
+                int sa = con[c].slot_a, sb = con[c].slot_b;
+                if (head[sa] == c && (sb < 0 || head[sb] == c)) {
+                    ContactRec r = con[c];
+                    constraint(B, r, fl, k, Cf, Cr);
+                    con[c].dln = r.dln;
+                    con[c].dlt1 = r.dlt1;
+                    con[c].dlt2 = r.dlt2;
+                    done[c] = 1;
+                } else {
+                    left = 1;
+                }
+            }
+        }
+        pending = __syncthreads_count(left);
+    }
+}
+
+// ------------------------------------------------------------------ plain path kernels
+__global__ void k_floor_setup(FloorDev *fl, float px, float py, float pz, float sx, float sy, float sz, float mass,
+                              float moi)
+{
+    if (threadIdx.x != 0) return;
+    M4 m = model_from_pose(mk3(px, py, pz), mk3(0.0f, 0.0f, 0.0f), mk3(sx, sy, sz));
+    V3 v[8];
+    model_vertices(m, v);
+    float r2 = 0.0f;
+    for (int k = 0; k < 8; ++k) {
+        fl->v[3 * k] = v[k].x;
+        fl->v[3 * k + 1] = v[k].y;
+        fl->v[3 * k + 2] = v[k].z;
+        V3 d = sub(v[k], mk3(px, py, pz));
+        r2 = fmaxf(r2, dot(d, d));
+    }
+    fl->c[0] = px;
+    fl->c[1] = py;
+    fl->c[2] = pz;
+    fl->brad = sqrtf(r2);
+    fl->mass = mass;
+    fl->moi = moi;
+}
+
+__global__ __launch_bounds__(256) void k_plain_forces(Planes B, float *verts, float *brad, int n, int nb, StepConsts c)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    V3 F = B.get3(F_FX, i), T = B.get3(F_TX, i);
+    integrate_forces_body(B, i, F, T, c);
+    B.set3(F_FX, i, mk3(0.0f, 0.0f, 0.0f));
+    B.set3(F_TX, i, mk3(0.0f, 0.0f, 0.0f));
+    brad[i] = i < nb ? box_vertices_body(B, i, verts + 24 * i) : B.at(F_SX, i);
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_plain_cull(Planes B, const float *brad, int nb, int ns, int64_t npairs,
+                                                    const FloorDev *flp, int *blk_counts, const int *blk_offsets,
+                                                    int *cand, int cap, unsigned long long *cnt)
+{
+    __shared__ int wave_tot[4];
+    __shared__ FloorDev fl;
+    if (threadIdx.x == 0) fl = *flp;
+    __syncthreads();
+    const int64_t start = (int64_t)blockIdx.x * CULL_CHUNK;
+    const int64_t end = min(start + CULL_CHUNK, npairs);
+    int count = 0;
+    int base_out = WRITE ? blk_offsets[blockIdx.x] : 0;
+    for (int64_t base = start; base < end; base += 256) {
+        int64_t p = base + threadIdx.x;
+        bool keep = false;
+        if (p < end) {
+            PairId q = decode_pair(p, nb, ns);
+            int sa, sb;
+            pair_slots(q, nb, sa, sb);
+            V3 ca, cb;
+            float ra, rb;
+            slot_sphere(B, brad, sa, fl, ca, ra);
+            slot_sphere(B, brad, sb, fl, cb, rb);
+            keep = cull_keep(ca, ra, cb, rb);
+        }
+        int total;
+        int r = block_rank<256>(keep, wave_tot, total);
+        if (WRITE && keep) {
+            int dst = base_out + count + r;
+            if (dst < cap) cand[dst] = (int)p;
+        }
+        count += total;
+    }
+    if (!WRITE && threadIdx.x == 0) blk_counts[blockIdx.x] = count;
+    if (WRITE && threadIdx.x == 0) atomicAdd(&cnt[CNT_PAIR_TESTS], (unsigned long long)(end - start));
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int *counts, int nblk, int *offsets, int *total, int cap,
+                                                       unsigned long long *cnt)
+{
+    __shared__ int wave_tot[SCAN_THREADS / 64];
+    int running = 0;
+    for (int base = 0; base < nblk; base += SCAN_THREADS) {
+        int i = base + threadIdx.x;
+        int v = i < nblk ? counts[i] : 0;
+        // inclusive scan within the block: wave scan then wave offsets
+        const int lane = lane_id(), w = threadIdx.x >> 6;
+        int x = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wave_tot[w] = x;
+        __syncthreads();
+        int woff = 0, tot = 0;
+        for (int k = 0; k < SCAN_THREADS / 64; ++k) {
+            int t = wave_tot[k];
+            woff += k < w ? t : 0;
+            tot += t;
+        }
+        if (i < nblk) offsets[i] = running + woff + x - v;
+        running += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *total = min(running, cap);
+        atomicAdd(&cnt[CNT_CANDIDATES], (unsigned long long)running);
+        if (running > cap) atomicAdd(&cnt[CNT_CAND_OVERFLOW], 1ull);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_plain_narrow(Planes B, const float *verts, int nb, int ns,
+                                                      const FloorDev *flp, const int *cand, const int *ncand,
+                                                      NarrowParams np, ContactRec *rec, int *flag,
+                                                      unsigned long long *cnt)
+{
+    __shared__ EpaLds lds[4];
+    __shared__ FloorDev fl;
+    if (threadIdx.x == 0) fl = *flp;
+    __syncthreads();
+    EpaLds &L = lds[threadIdx.x >> 6];
+    const int total = *ncand;
+    for (int base = blockIdx.x * 256; base < total; base += gridDim.x * 256) {
+        int c = base + threadIdx.x;
+        bool active = c < total;
+        ContactRec r;
+        bool hit = narrow_candidate(active, active ? cand[c] : 0, nb, ns, B, verts, fl, np, L, r, cnt);
+        if (active) {
+            rec[c] = r;
+            flag[c] = hit ? 1 : 0;
+        }
+    }
+}
+
+struct SolveArgs {
+    Planes B;
+    int n;
+    const int *flag;
+    const ContactRec *rec;
+    const int *ncand;
+    ContactRec *con;
+    int *done;
+    int con_cap;
+    StepConsts k;
+    const FloorDev *fl;
+    float Cf, Cr;
+    ContactRec *rec_out;  // recording (may be null)
+    int rec_cap;
+    int *rec_count;
+    unsigned long long *cnt;
+};
+
+// Ordered contact compaction, dependency-round solve and IntegrateVelocities of
+// one world, by one workgroup; head[] in dynamic LDS.
+__global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
+{
+    extern __shared__ int head[];
+    __shared__ int wave_tot[SOLVE_THREADS / 64];
+    __shared__ FloorDev fl;
+    if (threadIdx.x == 0) fl = *a.fl;
+    __syncthreads();
+    const int total = *a.ncand;
+    int ncon = 0;
+    for (int base = 0; base < total; base += SOLVE_THREADS) {
+        int c = base + threadIdx.x;
+        bool hit = c < total && a.flag[c];
+        int tot;
+        int r = block_rank<SOLVE_THREADS>(hit, wave_tot, tot);
+        if (hit && ncon + r < a.con_cap) a.con[ncon + r] = a.rec[c];
+        ncon += tot;
+    }
+    if (ncon > a.con_cap) {
+        if (threadIdx.x == 0) atomicAdd(&a.cnt[CNT_CON_OVERFLOW], 1ull);
+        ncon = a.con_cap;
+    }
+    if (threadIdx.x == 0) atomicAdd(&a.cnt[CNT_CONTACTS], (unsigned long long)ncon);
+    __syncthreads();
+    solve_rounds<SOLVE_THREADS>(a.B, a.n, a.con, ncon, a.done, head, fl, a.k, a.Cf, a.Cr);
+    for (int i = threadIdx.x; i < a.n; i += SOLVE_THREADS) integrate_velocities_body(a.B, i, a.k.dt);
+    if (a.rec_out) {
+        __syncthreads();
+        for (int c = threadIdx.x; c < ncon && c < a.rec_cap; c += SOLVE_THREADS) a.rec_out[c] = a.con[c];
+        if (threadIdx.x == 0) *a.rec_count = ncon;
+    }
+}
+
+// ------------------------------------------------------------------ batched kernel
+struct BatchArgs {
+    float *planes;          // [F_NFIELDS][N]
+    int64_t N;
+    const int *offsets;     // [nw+1]
+    const int *nboxes;      // [nw]
+    const int *order;       // largest world first
+    int nworlds;
+    int nmax;               // LDS stride (>= largest world)
+    int *queue;             // this step's work counter
+    StepConsts k;
+    const FloorDev *fl;
+    const float *wfric, *wrest;  // per-world coefficients (may be null)
+    int *cand;              // per-workgroup scratch, cap entries each
+    ContactRec *con;
+    int *done;
+    int cap;
+    ContactRec *rec_out;    // recording (may be null)
+    const int64_t *rec_off;
+    int *rec_count;
+    const int *rec_cap;
+    unsigned long long *cnt;
+};
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_batched_step(BatchArgs a)
+{
+    extern __shared__ float smem[];
+    constexpr int NW = BLOCK / 64;
+    __shared__ int wave_tot[NW];
+    __shared__ int s_job;
+    __shared__ FloorDev fl;
+    if (threadIdx.x == 0) fl = *a.fl;
+    const int tid = threadIdx.x;
+    const int nmax = a.nmax;
+    float *bodyL = smem;                               // [F_NLDS][nmax]
+    float *vertsL = bodyL + F_NLDS * nmax;             // [nmax][24]
+    float *bradL = vertsL + 24 * nmax;                 // [nmax]
+    int *headL = (int *)(bradL + nmax);                // [nmax]
+    EpaLds *epa = (EpaLds *)(headL + nmax + ((4 - (nmax & 3)) & 3));  // 16-byte aligned (nmax padded)
+    EpaLds &L = epa[tid >> 6];
+    int *cand = a.cand + (int64_t)blockIdx.x * a.cap;
+    ContactRec *con = a.con + (int64_t)blockIdx.x * a.cap;
+    int *done = a.done + (int64_t)blockIdx.x * a.cap;
+    const Planes G{a.planes, (int)a.N};
+
+    for (;;) {
+        if (tid == 0) s_job = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int job = s_job;
+        __syncthreads();
+        if (job >= a.nworlds) break;
+        const int w = a.order[job];
+        const int b0 = a.offsets[w];
+        const int n = a.offsets[w + 1] - b0;
+        const int nb = a.nboxes[w];
+        const int ns = n - nb;
+        const Planes Lb{bodyL, nmax};
+        const Planes Gw{a.planes + b0, (int)a.N};
+        // load the world, IntegrateForces, corners
+        for (int i = tid; i < n; i += BLOCK) {
+#pragma unroll
+            for (int f = 0; f < F_NLDS; ++f) Lb.at(f, i) = Gw.at(f, i);
+            V3 F = Gw.get3(F_FX, i), T = Gw.get3(F_TX, i);
+            integrate_forces_body(Lb, i, F, T, a.k);
+            Gw.set3(F_FX, i, mk3(0.0f, 0.0f, 0.0f));
+            Gw.set3(F_TX, i, mk3(0.0f, 0.0f, 0.0f));
+            bradL[i] = i < nb ? box_vertices_body(Lb, i, vertsL + 24 * i) : Lb.at(F_SX, i);
+        }
+        __syncthreads();
+        // cull over the canonical pairs, ordered
+        const int64_t npairs = pair_count(nb, ns);
+        int ncand = 0;
+        for (int64_t base = 0; base < npairs; base += BLOCK) {
+            int64_t p = base + tid;
+            bool keep = false;
+            if (p < npairs) {
+                PairId q = decode_pair(p, nb, ns);
+                int sa, sb;
+                pair_slots(q, nb, sa, sb);
+                V3 ca, cb;
+                float ra, rb;
+                slot_sphere(Lb, bradL, sa, fl, ca, ra);
+                slot_sphere(Lb, bradL, sb, fl, cb, rb);
+                keep = cull_keep(ca, ra, cb, rb);
+            }
+            int tot;
+            int r = block_rank<BLOCK>(keep, wave_tot, tot);
+            if (keep && ncand + r < a.cap) cand[ncand + r] = (int)p;
+            ncand += tot;
+        }
+        if (ncand > a.cap) {
+            if (tid == 0) atomicAdd(&a.cnt[CNT_CAND_OVERFLOW], 1ull);
+            ncand = a.cap;
+        }
+        if (tid == 0) {
+            atomicAdd(&a.cnt[CNT_PAIR_TESTS], (unsigned long long)npairs);
+            atomicAdd(&a.cnt[CNT_CANDIDATES], (unsigned long long)ncand);
+        }
+        __syncthreads();
+        // narrowphase over the candidates, contacts appended in order
+        int ncon = 0;
+        for (int base = 0; base < ncand; base += BLOCK) {
+            int c = base + tid;
+            bool active = c < ncand;
+            ContactRec r;
+            bool hit = narrow_candidate(active, active ? cand[c] : 0, nb, ns, Lb, vertsL, fl, a.k.np, L, r, a.cnt);
+            int tot;
+            int rk = block_rank<BLOCK>(hit, wave_tot, tot);
+            if (hit && ncon + rk < a.cap) con[ncon + rk] = r;
+            ncon += tot;
+        }
+        if (ncon > a.cap) {
+            if (tid == 0) atomicAdd(&a.cnt[CNT_CON_OVERFLOW], 1ull);
+            ncon = a.cap;
+        }
+        if (tid == 0) atomicAdd(&a.cnt[CNT_CONTACTS], (unsigned long long)ncon);
+        __syncthreads();
+        const float Cf = a.wfric ? a.wfric[w] : a.k.friction;
+        const float Cr = a.wrest ? a.wrest[w] : a.k.restitution;
+        solve_rounds<BLOCK>(Lb, n, con, ncon, done, headL, fl, a.k, Cf, Cr);
+        // IntegrateVelocities and write-back
+        for (int i = tid; i < n; i += BLOCK) {
+            integrate_velocities_body(Lb, i, a.k.dt);
+#pragma unroll
+            for (int f = 0; f < F_SX; ++f) Gw.at(f, i) = Lb.at(f, i);
+        }
+        if (a.rec_out) {
+            ContactRec *dst = a.rec_out + a.rec_off[w];
+            int rc = a.rec_cap[w];
+            for (int c = tid; c < ncon && c < rc; c += BLOCK) dst[c] = con[c];
+            if (tid == 0) a.rec_count[w] = ncon;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_i32(int *p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+}  // namespace nans
+
+// ====================================================================== host side
 struct nans_worlds {
-    int dummy;
+    int nworlds = 0;
+    int64_t N = 0;
+    int nmax = 0;
+    int64_t max_pairs = 0;
+    std::vector<int> h_off, h_nb;
+    std::vector<float> h_fric, h_rest;
+    float *planes = nullptr;
+    float *verts = nullptr;
+    float *brad = nullptr;
+    int *offsets = nullptr;
+    int *nboxes = nullptr;
+    int *order = nullptr;
+    float *wfric = nullptr;
+    float *wrest = nullptr;
+    FloorDev *floor = nullptr;
+    unsigned long long *cnt = nullptr;
+    // plain scratch
+    int cand_cap = 0;
+    int *cand = nullptr;
+    ContactRec *rec = nullptr;
+    int *flag = nullptr;
+    ContactRec *con = nullptr;
+    int *done = nullptr;
+    int *blk_counts = nullptr;
+    int *blk_offsets = nullptr;
+    int *ncand = nullptr;
+    int max_blk = 0;
+    // batched scratch
+    int b_grid = 0;
+    int b_cap = 0;
+    int *b_cand = nullptr;
+    ContactRec *b_con = nullptr;
+    int *b_done = nullptr;
+    int *queue = nullptr;
+    int queue_len = 0;
+    // recording
+    int record = 0;
+    ContactRec *rec_buf = nullptr;
+    int64_t *rec_off = nullptr;
+    int *rec_count = nullptr;
+    int *rec_cap = nullptr;
+    std::vector<int64_t> h_rec_off;
+    std::vector<int> h_rec_cap;
+    // graph cache of the batched step
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = -1;
+    int graph_block = 0;
+    nans_params graph_params;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    int tuned_block = 0;   // autotuner's choice of threads per world (0: not tuned)
 };
 
 namespace nans {
+
+static void wfree(void *p)
+{
+    if (p) hipFree(p);
+}
+
+static void release_batch_scratch(nans_worlds *W)
+{
+    wfree(W->b_cand);
+    wfree(W->b_con);
+    wfree(W->b_done);
+    W->b_cand = nullptr;
+    W->b_con = nullptr;
+    W->b_done = nullptr;
+    W->b_grid = 0;
+}
+
+static void release_recording(nans_worlds *W)
+{
+    wfree(W->rec_buf);
+    wfree(W->rec_off);
+    wfree(W->rec_count);
+    wfree(W->rec_cap);
+    W->rec_buf = nullptr;
+    W->rec_off = nullptr;
+    W->rec_count = nullptr;
+    W->rec_cap = nullptr;
+}
+
+static void drop_graph(nans_worlds *W)
+{
+    if (W->graph) hipGraphExecDestroy(W->graph);
+    W->graph = nullptr;
+    W->graph_steps = -1;
+}
+
 int worlds_release(nans_ctx *ctx)
 {
-    if (ctx && ctx->worlds) {
-        delete ctx->worlds;
-        ctx->worlds = nullptr;
+    if (!ctx || !ctx->worlds) return NANS_OK;
+    nans_worlds *W = ctx->worlds;
+    drop_graph(W);
+    for (void *p : {(void *)W->planes, (void *)W->verts, (void *)W->brad, (void *)W->offsets, (void *)W->nboxes,
+                    (void *)W->order, (void *)W->wfric, (void *)W->wrest, (void *)W->floor, (void *)W->cnt,
+                    (void *)W->cand, (void *)W->rec, (void *)W->flag, (void *)W->con, (void *)W->done,
+                    (void *)W->blk_counts, (void *)W->blk_offsets, (void *)W->ncand, (void *)W->queue})
+        wfree(p);
+    if (W->ev_in) hipEventDestroy(W->ev_in);
+    if (W->ev_out) hipEventDestroy(W->ev_out);
+    release_batch_scratch(W);
+    release_recording(W);
+    delete W;
+    ctx->worlds = nullptr;
+    return NANS_OK;
+}
+
+static StepConsts step_consts(const nans_params &p)
+{
+    StepConsts k;
+    k.np = narrow_params(p);
+    k.solver_iterations = p.solver_iterations;
+    k.beta = p.beta;
+    k.restitution = p.restitution;
+    k.friction = p.friction;
+    k.damping = p.damping;
+    k.gravity = p.gravity;
+    k.dt = p.dt;
+    return k;
+}
+
+static int setup_floor(nans_ctx *ctx, hipStream_t s)
+{
+    const nans_params &p = ctx->p;
+    k_floor_setup<<<1, 64, 0, s>>>(ctx->worlds->floor, p.floor_pos[0], p.floor_pos[1], p.floor_pos[2],
+                                   p.floor_scale[0], p.floor_scale[1], p.floor_scale[2], p.floor_mass, p.floor_moi);
+    NANS_HIP(hipGetLastError());
+    return NANS_OK;
+}
+
+template <typename T>
+static int dmalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    NANS_HIP(hipMalloc((void **)p, n * sizeof(T)));
+    return NANS_OK;
+}
+
+#define NANS_TRY(x)             \
+    do {                        \
+        int rc_ = (x);          \
+        if (rc_) return rc_;    \
+    } while (0)
+
+static int alloc_recording(nans_ctx *ctx)
+{
+    nans_worlds *W = ctx->worlds;
+    release_recording(W);
+    W->h_rec_off.assign(W->nworlds + 1, 0);
+    W->h_rec_cap.assign(W->nworlds, 0);
+    for (int w = 0; w < W->nworlds; ++w) {
+        int n = W->h_off[w + 1] - W->h_off[w];
+        W->h_rec_cap[w] = 16 * n + 64;
+        W->h_rec_off[w + 1] = W->h_rec_off[w] + W->h_rec_cap[w];
+    }
+    NANS_TRY(dmalloc(&W->rec_buf, (size_t)W->h_rec_off[W->nworlds]));
+    NANS_TRY(dmalloc(&W->rec_off, (size_t)W->nworlds));
+    NANS_TRY(dmalloc(&W->rec_count, (size_t)W->nworlds));
+    NANS_TRY(dmalloc(&W->rec_cap, (size_t)W->nworlds));
+    NANS_HIP(hipMemcpy(W->rec_off, W->h_rec_off.data(), sizeof(int64_t) * W->nworlds, hipMemcpyHostToDevice));
+    NANS_HIP(hipMemcpy(W->rec_cap, W->h_rec_cap.data(), sizeof(int) * W->nworlds, hipMemcpyHostToDevice));
+    std::vector<int> z(W->nworlds, 0);
+    NANS_HIP(hipMemcpy(W->rec_count, z.data(), sizeof(int) * W->nworlds, hipMemcpyHostToDevice));
+    return NANS_OK;
+}
+
+static size_t batch_lds_bytes(int nmax, int block)
+{
+    int pad = (4 - (nmax & 3)) & 3;
+    return sizeof(float) * ((size_t)(F_NLDS + 24 + 1) * nmax) + sizeof(int) * (size_t)(nmax + pad) +
+           sizeof(EpaLds) * (size_t)(block / 64);
+}
+
+}  // namespace nans
+
+extern "C" {
+
+int nans_upload_worlds(nans_ctx *ctx, int32_t nworlds, const int32_t *world_offsets, const nans_bodies_soa *b,
+                       const float *world_friction, const float *world_restitution)
+{
+    NANS_REQUIRE(ctx, NANS_ERR_INVALID, "ctx is NULL");
+    NANS_REQUIRE(nworlds >= 1, NANS_ERR_INVALID, "nworlds must be >= 1");
+    NANS_REQUIRE(world_offsets && b, NANS_ERR_INVALID, "world_offsets or bodies is NULL");
+    NANS_REQUIRE(world_offsets[0] == 0, NANS_ERR_INVALID, "world_offsets[0] must be 0");
+    int64_t N = world_offsets[nworlds];
+    NANS_REQUIRE(N >= 0 && N < (1LL << 30), NANS_ERR_CAPACITY, "body count out of range");
+    NANS_REQUIRE(N == 0 || (b->pos && b->vel && b->angvel && b->angles && b->scale && b->mass && b->moi && b->kind),
+                 NANS_ERR_INVALID, "a body array is NULL");
+    std::vector<int> off(world_offsets, world_offsets + nworlds + 1), nbx(nworlds);
+    int nmax = 0;
+    int64_t maxp = 0;
+    for (int w = 0; w < nworlds; ++w) {
+        NANS_REQUIRE(off[w + 1] >= off[w], NANS_ERR_INVALID, "world_offsets must be non-decreasing");
+        int n = off[w + 1] - off[w];
+        int nb = 0;
+        while (nb < n && b->kind[off[w] + nb] == NANS_SHAPE_BOX) ++nb;
+        for (int i = nb; i < n; ++i)
+            NANS_REQUIRE(b->kind[off[w] + i] == NANS_SHAPE_SPHERE, NANS_ERR_INVALID,
+                         "world %d: boxes must precede spheres and kinds must be box or sphere", w);
+        for (int i = 0; i < n; ++i)
+            NANS_REQUIRE(b->mass[off[w] + i] > 0.0f && b->moi[off[w] + i] > 0.0f, NANS_ERR_INVALID,
+                         "world %d body %d: mass and moi must be positive", w, i);
+        nbx[w] = nb;
+        nmax = std::max(nmax, n);
+        maxp = std::max(maxp, pair_count(nb, n - nb));
+    }
+    NANS_REQUIRE(nmax <= PLAIN_MAX_BODIES, NANS_ERR_CAPACITY, "a world holds %d bodies; the limit is %d", nmax,
+                 PLAIN_MAX_BODIES);
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    NANS_HIP(hipDeviceSynchronize());
+    worlds_release(ctx);
+    nans_worlds *W = new nans_worlds();
+    ctx->worlds = W;
+    W->nworlds = nworlds;
+    W->N = N;
+    W->nmax = nmax;
+    W->max_pairs = maxp;
+    W->h_off = off;
+    W->h_nb = nbx;
+    NANS_TRY(dmalloc(&W->planes, (size_t)F_NFIELDS * N));
+    NANS_TRY(dmalloc(&W->verts, (size_t)24 * N));
+    NANS_TRY(dmalloc(&W->brad, (size_t)N));
+    NANS_TRY(dmalloc(&W->offsets, (size_t)nworlds + 1));
+    NANS_TRY(dmalloc(&W->nboxes, (size_t)nworlds));
+    NANS_TRY(dmalloc(&W->order, (size_t)nworlds));
+    NANS_TRY(dmalloc(&W->floor, 1));
+    NANS_TRY(dmalloc(&W->cnt, CNT_N));
+    // host staging in plane layout
+    std::vector<float> h((size_t)F_NFIELDS * N, 0.0f);
+    auto put3 = [&](int f, const float *src) {
+        if (!src) return;
+        for (int c = 0; c < 3; ++c) std::copy(src + c * N, src + (c + 1) * N, h.begin() + (size_t)(f + c) * N);
+    };
+    put3(F_PX, b->pos);
+    put3(F_VX, b->vel);
+    put3(F_WX, b->angvel);
+    put3(F_AX, b->angles);
+    put3(F_SX, b->scale);
+    put3(F_FX, b->force);
+    put3(F_TX, b->torque);
+    std::copy(b->mass, b->mass + N, h.begin() + (size_t)F_MASS * N);
+    std::copy(b->moi, b->moi + N, h.begin() + (size_t)F_MOI * N);
+    NANS_HIP(hipMemcpy(W->planes, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+    NANS_HIP(hipMemcpy(W->offsets, off.data(), sizeof(int) * (nworlds + 1), hipMemcpyHostToDevice));
+    NANS_HIP(hipMemcpy(W->nboxes, nbx.data(), sizeof(int) * nworlds, hipMemcpyHostToDevice));
+    // largest world first (cost ~ pairs + bodies)
+    std::vector<int> ord(nworlds);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+        int nx = off[x + 1] - off[x], ny = off[y + 1] - off[y];
+        return nx > ny;
+    });
+    NANS_HIP(hipMemcpy(W->order, ord.data(), sizeof(int) * nworlds, hipMemcpyHostToDevice));
+    if (world_friction) {
+        W->h_fric.assign(world_friction, world_friction + nworlds);
+        NANS_TRY(dmalloc(&W->wfric, (size_t)nworlds));
+        NANS_HIP(hipMemcpy(W->wfric, world_friction, sizeof(float) * nworlds, hipMemcpyHostToDevice));
+    }
+    if (world_restitution) {
+        W->h_rest.assign(world_restitution, world_restitution + nworlds);
+        NANS_TRY(dmalloc(&W->wrest, (size_t)nworlds));
+        NANS_HIP(hipMemcpy(W->wrest, world_restitution, sizeof(float) * nworlds, hipMemcpyHostToDevice));
+    }
+    // plain scratch: candidates of the largest world
+    int64_t cap = std::min<int64_t>(maxp, 64LL * nmax + 4096);
+    W->cand_cap = (int)cap;
+    W->max_blk = (int)((maxp + CULL_CHUNK - 1) / CULL_CHUNK) + 1;
+    NANS_TRY(dmalloc(&W->cand, (size_t)cap));
+    NANS_TRY(dmalloc(&W->rec, (size_t)cap));
+    NANS_TRY(dmalloc(&W->flag, (size_t)cap));
+    NANS_TRY(dmalloc(&W->con, (size_t)cap));
+    NANS_TRY(dmalloc(&W->done, (size_t)cap));
+    NANS_TRY(dmalloc(&W->blk_counts, (size_t)W->max_blk));
+    NANS_TRY(dmalloc(&W->blk_offsets, (size_t)W->max_blk));
+    NANS_TRY(dmalloc(&W->ncand, 1));
+    NANS_HIP(hipFuncSetAttribute((const void *)k_plain_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(sizeof(int) * PLAIN_MAX_BODIES)));
+    k_zero_u64<<<1, 64>>>(W->cnt, CNT_N);
+    NANS_HIP(hipGetLastError());
+    NANS_TRY(setup_floor(ctx, nullptr));
+    if (W->record) NANS_TRY(alloc_recording(ctx));
+    NANS_HIP(hipDeviceSynchronize());
+    return NANS_OK;
+}
+
+int nans_download_state(nans_ctx *ctx, nans_bodies_soa *b)
+{
+    NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
+    NANS_REQUIRE(b, NANS_ERR_INVALID, "bodies is NULL");
+    nans_worlds *W = ctx->worlds;
+    int64_t N = W->N;
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    NANS_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<float> h((size_t)F_NFIELDS * N);
+    NANS_HIP(hipMemcpy(h.data(), W->planes, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+    auto get3 = [&](int f, float *dst) {
+        if (!dst) return;
+        for (int c = 0; c < 3; ++c)
+            std::copy(h.begin() + (size_t)(f + c) * N, h.begin() + (size_t)(f + c + 1) * N, dst + c * N);
+    };
+    get3(F_PX, b->pos);
+    get3(F_VX, b->vel);
+    get3(F_WX, b->angvel);
+    get3(F_AX, b->angles);
+    get3(F_FX, b->force);
+    get3(F_TX, b->torque);
+    return NANS_OK;
+}
+
+int nans_record_contacts(nans_ctx *ctx, int32_t enable)
+{
+    NANS_REQUIRE(ctx, NANS_ERR_INVALID, "ctx is NULL");
+    if (!ctx->worlds) {
+        NANS_REQUIRE(!enable, NANS_ERR_STATE, "upload worlds before enabling contact recording");
+        return NANS_OK;
+    }
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    NANS_HIP(hipDeviceSynchronize());
+    nans_worlds *W = ctx->worlds;
+    W->record = enable ? 1 : 0;
+    drop_graph(W);
+    if (W->record) return alloc_recording(ctx);
+    release_recording(W);
+    return NANS_OK;
+}
+
+int nans_get_contacts(nans_ctx *ctx, int32_t world, nans_contact *out, int32_t max_contacts, int32_t *n)
+{
+    NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
+    nans_worlds *W = ctx->worlds;
+    NANS_REQUIRE(W->record, NANS_ERR_STATE, "contact recording is off (nans_record_contacts)");
+    NANS_REQUIRE(world >= 0 && world < W->nworlds, NANS_ERR_INVALID, "world %d out of range", world);
+    NANS_REQUIRE(n, NANS_ERR_INVALID, "n is NULL");
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    NANS_HIP(hipStreamSynchronize(ctx->stream));
+    int cnt = 0;
+    NANS_HIP(hipMemcpy(&cnt, W->rec_count + world, sizeof(int), hipMemcpyDeviceToHost));
+    NANS_REQUIRE(cnt <= W->h_rec_cap[world], NANS_ERR_CAPACITY, "world %d had %d contacts; %d were recorded", world,
+                 cnt, W->h_rec_cap[world]);
+    *n = cnt;
+    int m = std::min(cnt, max_contacts);
+    if (m > 0 && out) {
+        std::vector<ContactRec> tmp(m);
+        NANS_HIP(hipMemcpy(tmp.data(), W->rec_buf + W->h_rec_off[world], sizeof(ContactRec) * m,
+                           hipMemcpyDeviceToHost));
+        for (int i = 0; i < m; ++i) {
+            out[i].type = tmp[i].type;
+            out[i].index_a = tmp[i].ia;
+            out[i].index_b = tmp[i].ib;
+            out[i].pad = 0;
+            for (int c = 0; c < 3; ++c) {
+                out[i].point_a[c] = tmp[i].pa[c];
+                out[i].point_b[c] = tmp[i].pb[c];
+                out[i].normal[c] = tmp[i].n[c];
+            }
+            out[i].dl_normal = tmp[i].dln;
+            out[i].dl_tangent1 = tmp[i].dlt1;
+            out[i].dl_tangent2 = tmp[i].dlt2;
+        }
     }
     return NANS_OK;
 }
+
+int nans_get_stats(nans_ctx *ctx, nans_stats *st)
+{
+    NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
+    NANS_REQUIRE(st, NANS_ERR_INVALID, "stats is NULL");
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    NANS_HIP(hipStreamSynchronize(ctx->stream));
+    unsigned long long c[CNT_N];
+    NANS_HIP(hipMemcpy(c, ctx->worlds->cnt, sizeof(c), hipMemcpyDeviceToHost));
+    st->pair_tests = (int64_t)c[CNT_PAIR_TESTS];
+    st->candidates = (int64_t)c[CNT_CANDIDATES];
+    st->gjk_found = (int64_t)c[CNT_GJK_FOUND];
+    st->contacts = (int64_t)c[CNT_CONTACTS];
+    st->epa_overflow = (int64_t)c[CNT_EPA_OVERFLOW];
+    st->capacity_overflow = (int64_t)(c[CNT_CAND_OVERFLOW] + c[CNT_CON_OVERFLOW]);
+    return NANS_OK;
+}
+
+int nans_reset_stats(nans_ctx *ctx)
+{
+    NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    k_zero_u64<<<1, 64, 0, ctx->stream>>>(ctx->worlds->cnt, CNT_N);
+    NANS_HIP(hipGetLastError());
+    NANS_HIP(hipStreamSynchronize(ctx->stream));
+    return NANS_OK;
+}
+
+static int check_overflow(nans_ctx *ctx)
+{
+    unsigned long long c[CNT_N];
+    NANS_HIP(hipMemcpy(c, ctx->worlds->cnt, sizeof(c), hipMemcpyDeviceToHost));
+    NANS_REQUIRE(c[CNT_CAND_OVERFLOW] == 0 && c[CNT_CON_OVERFLOW] == 0, NANS_ERR_CAPACITY,
+                 "pair candidates or contacts exceeded the scratch capacity");
+    return NANS_OK;
+}
+
+int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
+{
+    NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
+    NANS_REQUIRE(nsteps >= 0, NANS_ERR_INVALID, "nsteps < 0");
+    nans_worlds *W = ctx->worlds;
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    hipStream_t s = ctx->stream;
+    NANS_TRY(setup_floor(ctx, s));
+    const StepConsts k = step_consts(ctx->p);
+    for (int step = 0; step < nsteps; ++step) {
+        for (int w = 0; w < W->nworlds; ++w) {
+            const int b0 = W->h_off[w], n = W->h_off[w + 1] - b0, nb = W->h_nb[w], ns = n - nb;
+            if (n == 0) continue;
+            Planes B{W->planes + b0, (int)W->N};
+            float *verts = W->verts + (int64_t)24 * b0;
+            float *brad = W->brad + b0;
+            const int64_t npairs = pair_count(nb, ns);
+            const int nblk = (int)((npairs + CULL_CHUNK - 1) / CULL_CHUNK);
+            k_plain_forces<<<(n + 255) / 256, 256, 0, s>>>(B, verts, brad, n, nb, k);
+            if (nblk > 0) {
+                k_plain_cull<false><<<nblk, 256, 0, s>>>(B, brad, nb, ns, npairs, W->floor, W->blk_counts, nullptr,
+                                                         nullptr, 0, W->cnt);
+                k_scan<<<1, SCAN_THREADS, 0, s>>>(W->blk_counts, nblk, W->blk_offsets, W->ncand, W->cand_cap, W->cnt);
+                k_plain_cull<true><<<nblk, 256, 0, s>>>(B, brad, nb, ns, npairs, W->floor, nullptr, W->blk_offsets,
+                                                        W->cand, W->cand_cap, W->cnt);
+                int grid = std::max(1, std::min((W->cand_cap + 255) / 256, 2048));
+                k_plain_narrow<<<grid, 256, 0, s>>>(B, verts, nb, ns, W->floor, W->cand, W->ncand, k.np, W->rec,
+                                                    W->flag, W->cnt);
+            } else {
+                k_zero_i32<<<1, 64, 0, s>>>(W->ncand, 1);
+            }
+            SolveArgs a;
+            a.B = B;
+            a.n = n;
+            a.flag = W->flag;
+            a.rec = W->rec;
+            a.ncand = W->ncand;
+            a.con = W->con;
+            a.done = W->done;
+            a.con_cap = W->cand_cap;
+            a.k = k;
+            a.fl = W->floor;
+            a.Cf = W->h_fric.empty() ? k.friction : W->h_fric[w];
+            a.Cr = W->h_rest.empty() ? k.restitution : W->h_rest[w];
+            a.rec_out = W->record ? W->rec_buf + W->h_rec_off[w] : nullptr;
+            a.rec_cap = W->record ? W->h_rec_cap[w] : 0;
+            a.rec_count = W->record ? W->rec_count + w : nullptr;
+            a.cnt = W->cnt;
+            k_plain_solve<<<1, SOLVE_THREADS, sizeof(int) * n, s>>>(a);
+            NANS_HIP(hipGetLastError());
+        }
+    }
+    NANS_HIP(hipStreamSynchronize(s));
+    return check_overflow(ctx);
+}
+
+}  // extern "C"
+
+namespace nans {
+
+template <int BLOCK>
+static int batched_config(nans_ctx *ctx, int *grid, size_t *lds)
+{
+    nans_worlds *W = ctx->worlds;
+    *lds = batch_lds_bytes(W->nmax, BLOCK);
+    NANS_HIP(hipFuncSetAttribute((const void *)k_batched_step<BLOCK>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)*lds));
+    int per_cu = 0;
+    NANS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batched_step<BLOCK>, BLOCK, *lds));
+    NANS_REQUIRE(per_cu > 0, NANS_ERR_CAPACITY, "a world of %d bodies does not fit the LDS of one workgroup",
+                 W->nmax);
+    int cus = 0;
+    NANS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->p.device));
+    *grid = std::max(1, std::min(W->nworlds, per_cu * cus));
+    return NANS_OK;
+}
+
+template <int BLOCK>
+static int batched_capture(nans_ctx *ctx, int nsteps, hipStream_t cs)
+{
+    nans_worlds *W = ctx->worlds;
+    int grid = 0;
+    size_t lds = 0;
+    NANS_TRY(batched_config<BLOCK>(ctx, &grid, &lds));
+    int cap = (int)std::min<int64_t>(W->max_pairs, 32LL * W->nmax + 1024);
+    cap = std::max(cap, 64);
+    if (grid > W->b_grid || cap != W->b_cap) {
+        release_batch_scratch(W);
+        NANS_TRY(dmalloc(&W->b_cand, (size_t)grid * cap));
+        NANS_TRY(dmalloc(&W->b_con, (size_t)grid * cap));
+        NANS_TRY(dmalloc(&W->b_done, (size_t)grid * cap));
+        W->b_grid = grid;
+        W->b_cap = cap;
+    }
+    if (nsteps > W->queue_len) {
+        wfree(W->queue);
+        W->queue = nullptr;
+        NANS_TRY(dmalloc(&W->queue, (size_t)nsteps));
+        W->queue_len = nsteps;
+    }
+    BatchArgs a;
+    a.planes = W->planes;
+    a.N = W->N;
+    a.offsets = W->offsets;
+    a.nboxes = W->nboxes;
+    a.order = W->order;
+    a.nworlds = W->nworlds;
+    a.nmax = W->nmax;
+    a.k = step_consts(ctx->p);
+    a.fl = W->floor;
+    a.wfric = W->wfric;
+    a.wrest = W->wrest;
+    a.cand = W->b_cand;
+    a.con = W->b_con;
+    a.done = W->b_done;
+    a.cap = W->b_cap;
+    a.rec_out = W->record ? W->rec_buf : nullptr;
+    a.rec_off = W->rec_off;
+    a.rec_count = W->rec_count;
+    a.rec_cap = W->rec_cap;
+    a.cnt = W->cnt;
+    hipGraph_t g = nullptr;
+    NANS_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    k_zero_i32<<<(nsteps + 255) / 256, 256, 0, cs>>>(W->queue, nsteps);
+    for (int s = 0; s < nsteps; ++s) {
+        a.queue = W->queue + s;
+        k_batched_step<BLOCK><<<grid, BLOCK, lds, cs>>>(a);
+    }
+    hipError_t e = hipStreamEndCapture(cs, &g);
+    NANS_REQUIRE(e == hipSuccess, NANS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    drop_graph(W);
+    e = hipGraphInstantiate(&W->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    NANS_REQUIRE(e == hipSuccess, NANS_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    W->graph_steps = nsteps;
+    W->graph_block = BLOCK;
+    W->graph_params = ctx->p;
+    return NANS_OK;
+}
+
+int batched_run(nans_ctx *ctx, int nsteps, int block)
+{
+    nans_worlds *W = ctx->worlds;
+    NANS_REQUIRE(W->nmax <= BATCH_MAX_BODIES, NANS_ERR_CAPACITY,
+                 "the batched step keeps a world in LDS: at most %d bodies per world (largest: %d)",
+                 BATCH_MAX_BODIES, W->nmax);
+    hipStream_t cs = ctx->own_stream;
+    if (!W->ev_in) {
+        NANS_HIP(hipEventCreateWithFlags(&W->ev_in, hipEventDisableTiming));
+        NANS_HIP(hipEventCreateWithFlags(&W->ev_out, hipEventDisableTiming));
+    }
+    bool reuse = W->graph && W->graph_steps == nsteps && W->graph_block == block &&
+                 memcmp(&W->graph_params, &ctx->p, sizeof(nans_params)) == 0;
+    if (!reuse) {
+        NANS_TRY(setup_floor(ctx, ctx->stream));
+        NANS_HIP(hipStreamSynchronize(ctx->stream));
+        switch (block) {
+        case 64: NANS_TRY(batched_capture<64>(ctx, nsteps, cs)); break;
+        case 128: NANS_TRY(batched_capture<128>(ctx, nsteps, cs)); break;
+        case 512: NANS_TRY(batched_capture<512>(ctx, nsteps, cs)); break;
+        default: NANS_TRY(batched_capture<256>(ctx, nsteps, cs)); break;
+        }
+    }
+    NANS_HIP(hipEventRecord(W->ev_in, ctx->stream));
+    NANS_HIP(hipStreamWaitEvent(cs, W->ev_in, 0));
+    NANS_HIP(hipGraphLaunch(W->graph, cs));
+    NANS_HIP(hipEventRecord(W->ev_out, cs));
+    NANS_HIP(hipStreamWaitEvent(ctx->stream, W->ev_out, 0));
+    return NANS_OK;
+}
+
 }  // namespace nans
+
+extern "C" {
+
+int nans_step_batched(nans_ctx *ctx, int32_t nsteps)
+{
+    NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
+    NANS_REQUIRE(nsteps >= 0, NANS_ERR_INVALID, "nsteps < 0");
+    if (nsteps == 0) return NANS_OK;
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    nans_worlds *W = ctx->worlds;
+    int block = W->tuned_block ? W->tuned_block : 256;
+    NANS_TRY(batched_run(ctx, nsteps, block));
+    NANS_HIP(hipStreamSynchronize(ctx->stream));
+    return check_overflow(ctx);
+}
+
+}  // extern "C"

@@ -12,9 +12,12 @@
 namespace nans {
 
 // Field planes of a body view.
+// The first F_NLDS fields are the ones the batched path keeps in LDS.
 enum : int {
     F_PX = 0, F_PY, F_PZ, F_VX, F_VY, F_VZ, F_WX, F_WY, F_WZ, F_AX, F_AY, F_AZ,
-    F_FX, F_FY, F_FZ, F_TX, F_TY, F_TZ, F_SX, F_SY, F_SZ, F_MASS, F_MOI,
+    F_SX, F_SY, F_SZ, F_MASS, F_MOI,
+    F_NLDS,
+    F_FX = F_NLDS, F_FY, F_FZ, F_TX, F_TY, F_TZ,
     F_NFIELDS
 };
 
@@ -73,17 +76,14 @@ __device__ inline V3 rk4(float dt, V3 y0, V3 F, float m, const StepConsts &c)
     return add(y0, mul(sum, 1.0f / 6.0f));
 }
 
-// IntegrateForces for one body (code/nans.cpp:974-1018): RK4 on V and W, forces
-// cleared.
-__device__ inline void integrate_forces_body(const Planes &B, int i, const StepConsts &c)
+// IntegrateForces for one body (code/nans.cpp:974-1018): RK4 on V and W with the
+// force and torque F, T (cleared by the caller).
+__device__ inline void integrate_forces_body(const Planes &B, int i, V3 F, V3 T, const StepConsts &c)
 {
     V3 V = B.get3(F_VX, i), W = B.get3(F_WX, i);
-    V3 F = B.get3(F_FX, i), T = B.get3(F_TX, i);
     float m = B.at(F_MASS, i), I = B.at(F_MOI, i);
     B.set3(F_VX, i, rk4<false>(c.dt, V, F, m, c));
     B.set3(F_WX, i, rk4<true>(c.dt, W, T, I, c));
-    B.set3(F_FX, i, mk3(0.0f, 0.0f, 0.0f));
-    B.set3(F_TX, i, mk3(0.0f, 0.0f, 0.0f));
 }
 
 // IntegrateVelocities for one body (code/nans.cpp:1331-1349)
@@ -102,15 +102,19 @@ __device__ inline float box_vertices_body(const Planes &B, int i, float *out24)
     model_vertices(m, v);
     V3 P = B.get3(F_PX, i);
     float r2 = 0.0f;
+    bool bad = false;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         out24[3 * k] = v[k].x;
         out24[3 * k + 1] = v[k].y;
         out24[3 * k + 2] = v[k].z;
         V3 d = sub(v[k], P);
-        r2 = fmaxf(r2, dot(d, d));
+        float d2 = dot(d, d);
+        bad |= d2 != d2;
+        r2 = fmaxf(r2, d2);
     }
-    return sqrtf(r2);
+    // a NaN corner makes the radius NaN, which keeps every pair of the body
+    return bad ? __int_as_float(0x7fc00000) : sqrtf(r2);
 }
 
 // ------------------------------------------------------------------ pairs (code/nans.cpp:1355-1535)

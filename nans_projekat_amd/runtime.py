@@ -55,6 +55,21 @@ class NarrowOut(ctypes.Structure):
                 ("status", ctypes.c_void_p)]
 
 
+class BodiesSoa(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("pos", "vel", "angvel", "angles", "force", "torque", "scale", "mass",
+                                               "moi", "kind")]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in ("pair_tests", "candidates", "gjk_found", "contacts", "epa_overflow",
+                                              "capacity_overflow")]
+
+
+# nans_contact (include/nans_gpu.h) as a numpy record
+CONTACT_DTYPE = np.dtype([("type", "<i4"), ("index_a", "<i4"), ("index_b", "<i4"), ("pad", "<i4"),
+                          ("point_a", "<f4", 3), ("point_b", "<f4", 3), ("normal", "<f4", 3),
+                          ("dl_normal", "<f4"), ("dl_tangent1", "<f4"), ("dl_tangent2", "<f4")])
+
 _lib = None
 
 
@@ -94,6 +109,10 @@ def to_params(sp: "_scenes.StepParams | None" = None, device: int = 0, **overrid
 
 def _ptr(t) -> ctypes.c_void_p:
     return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _np(a) -> ctypes.c_void_p:
+    return ctypes.c_void_p(0 if a is None else a.ctypes.data)
 
 
 class Context:
@@ -185,3 +204,58 @@ class Context:
             a = v.cpu().numpy()
             res[k] = np.ascontiguousarray(a.T) if a.ndim == 2 else a
         return res
+
+    # ------------------------------------------------------------------ worlds
+    def upload(self, scene: "_scenes.Scene") -> None:
+        """Copy a batch of worlds (scenes.Scene) to the device."""
+        planes = lambda a: np.ascontiguousarray(np.asarray(a, np.float32).T)
+        self._host = dict(pos=planes(scene.pos), vel=planes(scene.vel), angvel=planes(scene.angvel),
+                          angles=planes(scene.angles), force=planes(scene.force), torque=planes(scene.torque),
+                          scale=planes(scene.scale), mass=np.ascontiguousarray(scene.mass, np.float32),
+                          moi=np.ascontiguousarray(scene.moi, np.float32),
+                          kind=np.ascontiguousarray(scene.kind, np.int32))
+        self._offsets = np.ascontiguousarray(scene.offsets, np.int32)
+        fr = None if scene.world_friction is None else np.ascontiguousarray(scene.world_friction, np.float32)
+        rs = None if scene.world_restitution is None else np.ascontiguousarray(scene.world_restitution, np.float32)
+        self._coeffs = (fr, rs)
+        b = BodiesSoa(*[_np(self._host[k]) for k in ("pos", "vel", "angvel", "angles", "force", "torque", "scale",
+                                                     "mass", "moi", "kind")])
+        self.torch.cuda.synchronize(self.device)
+        check(self.lib.nans_upload_worlds(self.h, int(scene.nworlds), _np(self._offsets), ctypes.byref(b),
+                                          _np(fr), _np(rs)))
+        self.nworlds = int(scene.nworlds)
+        self.nbodies = int(scene.nbodies)
+
+    def download(self) -> dict:
+        """pos, vel, angvel, angles, force, torque as [N][3] float32 arrays."""
+        n = self.nbodies
+        out = {k: np.zeros((3, n), np.float32) for k in ("pos", "vel", "angvel", "angles", "force", "torque")}
+        b = BodiesSoa(*[_np(out.get(k)) for k in ("pos", "vel", "angvel", "angles", "force", "torque")],
+                      None, None, None, None)
+        check(self.lib.nans_download_state(self.h, ctypes.byref(b)))
+        return {k: np.ascontiguousarray(v.T) for k, v in out.items()}
+
+    def step_plain(self, nsteps: int = 1) -> None:
+        self._sync_stream()
+        check(self.lib.nans_step_plain(self.h, int(nsteps)))
+
+    def step_batched(self, nsteps: int = 1) -> None:
+        self._sync_stream()
+        check(self.lib.nans_step_batched(self.h, int(nsteps)))
+
+    def record_contacts(self, enable: bool = True) -> None:
+        check(self.lib.nans_record_contacts(self.h, int(bool(enable))))
+
+    def contacts(self, world: int = 0, max_contacts: int = 1 << 16) -> np.ndarray:
+        buf = np.zeros(max_contacts, CONTACT_DTYPE)
+        n = ctypes.c_int32(0)
+        check(self.lib.nans_get_contacts(self.h, int(world), _np(buf), int(max_contacts), ctypes.byref(n)))
+        return buf[: min(n.value, max_contacts)].copy()
+
+    def stats(self) -> dict:
+        st = Stats()
+        check(self.lib.nans_get_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def reset_stats(self) -> None:
+        check(self.lib.nans_reset_stats(self.h))

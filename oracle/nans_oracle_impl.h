@@ -19,8 +19,15 @@
 
 #if ORA_F32
 #define SQRT sqrtf
-#define SIN sinf
-#define COS cosf
+/* glm::rotate calls the binary32 cos/sin of the linked libm (code/nans.cpp:1915-1920
+ * through matrix_transform.inl).  Those are implementation-defined to within an ulp
+ * (glibc's sinf/cosf are not correctly rounded for about 1% of arguments), so the
+ * restatement uses the correctly rounded binary32 value, obtained by rounding the
+ * binary64 result. */
+static inline float ora_sinf(float x) { return (float)sin((double)x); }
+static inline float ora_cosf(float x) { return (float)cos((double)x); }
+#define SIN ora_sinf
+#define COS ora_cosf
 #else
 #define SQRT sqrt
 #define SIN sin
