@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X simulation step on one GPU.
+
+Headline: config 3 -- 4096 independent worlds of 8..256 bodies (log-uniform),
+box stacks hit by 1-4 spheres, dt = 1/120, 10 impulse iterations -- advanced
+by the batched step (one persistent launch per step, replayed from a hipGraph).
+A "step" is one step of all 4096 worlds; value = world-body-steps/s over the K
+timed steps, with the state resident in HBM.  The timed region starts from
+the scene's initial state (it is restored after the warmup), so K = 240 is
+exactly the configured rollout.
+
+Also reported on the same line:
+  plain_gpu       the plain step (one world per launch sequence, the drop-in
+                  path) on a size-stratified subset of the same worlds, and the
+                  batched/plain ratio;
+  narrowphase     config 2 (1M box/sphere pairs, GJK + EPA) pairs/s;
+  roofline        the batched step kernel: algorithmic FP32 flops per launch
+                  (event counts frozen in tests/golden/work_counts.json times the
+                  per-event weights below) over its average launch time, timed
+                  with HIP events on the stream it runs on;
+  cpu_baseline    the CPU oracle (binary32 restatement of the reference step,
+                  brute-force all pairs as the reference does) on a bounded
+                  sample of config-3 worlds, one process per core.
+
+Usage: python bench.py [--gpus 1] [--steps 240] [--warmup 5]
+"""
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "world-body-steps/s + GJK/EPA pairs/s, batched vs plain GPU step, at stated tol"
+PEAK_FP32_TFLOPS = 157.3   # MI355X vector FP32 peak (the kernels issue no MFMA)
+PEAK_HBM_GBS = 8000.0
+
+# Algorithmic FP32 flops per counted event (mul/add/sub/div/sqrt = 1, compares 0),
+# for the formulation of the HIP kernels (glm-order arithmetic, EPA face distances
+# kept per face).  Event names are those of the oracle's work counters.
+FLOP_WEIGHTS = {
+    "body_box": 757,         # RK4 206 + Model rebuild 250 + 8 corners 224 + IntegrateVelocities 12 + radius 65
+    "body_sphere": 218,      # RK4 206 + IntegrateVelocities 12
+    "pairs": 21,             # bounding-sphere cull test
+    "evolve0": 19,           # EvolveSimplex base (direction from the centres) + empty-simplex case
+    "evolve1": 22,
+    "evolve2": 68,           # segment case
+    "evolve3": 89,           # triangle case
+    "evolve4": 73,           # tetrahedron case
+    "evolve_length": 17,     # termination length + AddSupport combine and test
+    "support_box": 40,       # 8 dot products
+    "support_sphere": 16,    # normalize + scale + add
+    "epa_iter": 12,          # support combine + convergence test
+    "epa_faces": 8,          # visibility test per face
+    "epa_edges": 38,         # new face per horizon edge: normal, normalize, distance, flip
+    "epa_final": 85,         # barycentric and contact points
+    "constraints": 400,      # Jacobians, effective masses, impulse application
+    "solver_iterations": 14, # one accumulate/clamp iteration
+}
+STATE_BYTES_PER_BODY_STEP = 164  # read P,V,W,A,S,m,I,F,T (92 B), write P,V,W,A and cleared F,T (72 B)
+
+
+def algorithmic_flops(counts):
+    return sum(FLOP_WEIGHTS[k] * counts.get(k, 0) for k in FLOP_WEIGHTS)
+
+
+def _cpu_init():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    O.lib()
+
+
+def _cpu_world(args):
+    world, nsteps = args
+    import oracle as O
+
+    ws = O.WorldState(world, 0, "f32")
+    t = time.perf_counter()
+    ws.run(nsteps)
+    return world.nbodies * nsteps, time.perf_counter() - t
+
+
+def cpu_baseline(sc, seconds, procs, nsteps):
+    """Oracle on a size-stratified sample of config-3 worlds, nsteps each, one
+    world per task on `procs` processes; stops taking worlds after `seconds`."""
+    import numpy as np
+
+    order = np.argsort(np.diff(sc.offsets))
+    sample = [int(order[i]) for i in np.linspace(0, len(order) - 1, 48).astype(int)]
+    tasks = [(sc.world(w), nsteps) for w in sample]
+    body_steps, busy = 0, 0.0
+    with mp.get_context("spawn").Pool(procs, initializer=_cpu_init) as pool:
+        pool.map(abs, range(procs))  # start every worker before the clock
+        t0 = time.perf_counter()
+        for bs, dt in pool.imap_unordered(_cpu_world, tasks):
+            body_steps += bs
+            busy += dt
+            if time.perf_counter() - t0 > seconds:
+                pool.terminate()
+                break
+    wall = time.perf_counter() - t0
+    return {"value": body_steps / wall, "unit": "world-body-steps/s", "cores": procs, "kind": "port",
+            "sample": "%d body-steps: size-stratified config-3 worlds x %d steps, all-pairs GJK/EPA "
+                      "(binary32 oracle, gcc -O2), %.1f s wall, %.1f s CPU" % (body_steps, nsteps, wall, busy)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=240)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--plain-worlds", type=int, default=64)
+    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--np-reps", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-procs", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--headline-only", action="store_true", help="only the batched config-3 measurement")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build runs no multi-GPU path (requested --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from nans_projekat_amd import runtime
+    from nans_projekat_amd import scenes as S
+
+    t_gen = time.perf_counter()
+    sc = S.config3()
+    print("config 3: %d worlds, %d bodies (generated in %.1f s)" % (sc.nworlds, sc.nbodies,
+                                                                    time.perf_counter() - t_gen), file=sys.stderr)
+    ctx = runtime.Context(sc.params)
+    ctx.upload(sc)
+    ctx.step_batched(max(1, args.warmup))
+    ctx.set_state(sc)
+    ctx.reset_stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.step_batched(args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    timing = ctx.batched_timing()
+    stats = ctx.stats()
+    wall = t1 - t0
+    value = sc.nbodies * args.steps / wall
+    kernel_ms = timing["elapsed_ms"] / args.steps
+    print("batched: %.3f s for %d steps (%.3f ms/step, kernel %.3f ms)" % (wall, args.steps, 1e3 * wall / args.steps,
+                                                                          kernel_ms), file=sys.stderr)
+
+    roof = None
+    counts_path = os.path.join(ROOT, "tests", "golden", "work_counts.json")
+    if os.path.exists(counts_path):
+        with open(counts_path) as f:
+            c3 = json.load(f)["config3_batched_240_steps"]
+        flops = algorithmic_flops(c3) * args.steps / c3["steps"]
+        tflops = flops / args.steps / (kernel_ms * 1e-3) / 1e12
+        gbs = STATE_BYTES_PER_BODY_STEP * sc.nbodies / (kernel_ms * 1e-3) / 1e9
+        roof = {"bound": "mfma", "achieved": round(tflops, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tflops / PEAK_FP32_TFLOPS, 5), "traffic": None,
+                "kernel": "k_batched_step", "compute": "vector FP32 (VALU); the kernels issue no MFMA, the "
+                "peak is MI355X's FP32 vector/matrix figure",
+                "flops_per_launch": round(flops / args.steps), "avg_launch_ms": round(kernel_ms, 5),
+                "flops_exact_workload": args.steps == c3["steps"],
+                "hbm": {"achieved": round(gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(gbs / PEAK_HBM_GBS, 6), "bytes_per_body_step": STATE_BYTES_PER_BODY_STEP}}
+
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "world-body-steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic: seeded config-3 scenes (SplitMix64, seed 0x4E414E53)",
+           "config": {"workload": "config3: 4096 worlds, 8-256 bodies (log-uniform), box stacks + 1-4 spheres, "
+                                  "dt 1/120, RK4, 10 impulse iterations, batched step",
+                      "worlds": sc.nworlds, "bodies": sc.nbodies, "parallelism": "none (1 GPU)",
+                      "threads_per_world": timing["threads_per_world"], "workgroups": timing["workgroups"],
+                      "lds_bytes": timing["lds_bytes"]},
+           "roofline": roof,
+           "step_counts": {k: stats[k] for k in ("pair_tests", "candidates", "gjk_found", "contacts")},
+           "tolerance": "bit-exact with the binary32 CPU restatement of the reference step (tests/)"}
+
+    if not args.headline_only:
+        # plain path on a size-stratified subset of the same worlds
+        order = np.argsort(np.diff(sc.offsets))
+        pick = sorted(set(int(order[i]) for i in np.linspace(0, len(order) - 1, args.plain_worlds).astype(int)))
+        sub = sc.subset(pick)
+        pctx = runtime.Context(sub.params)
+        pctx.upload(sub)
+        pctx.step_plain(1)
+        pctx.set_state(sub)
+        torch.cuda.synchronize()
+        p0 = time.perf_counter()
+        pctx.step_plain(args.plain_steps)
+        torch.cuda.synchronize()
+        pwall = time.perf_counter() - p0
+        plain = sub.nbodies * args.plain_steps / pwall
+        pctx.close()
+        out["plain_gpu"] = {"value": round(plain, 1), "unit": "world-body-steps/s",
+                            "sample": "%d size-stratified config-3 worlds x %d steps" % (len(pick), args.plain_steps)}
+        out["batched_vs_plain"] = round(value / plain, 2)
+        # narrowphase microbench (config 2)
+        pairs = S.config2_pairs()
+        nctx = runtime.Context(S.StepParams())
+        dp = nctx.upload_pairs(pairs)
+        o = nctx.narrowphase(dp)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        n0 = time.perf_counter()
+        e0.record()
+        for _ in range(args.np_reps):
+            nctx.narrowphase(dp, o)
+        e1.record()
+        torch.cuda.synchronize()
+        nwall = time.perf_counter() - n0
+        n = len(pairs["type_a"])
+        out["narrowphase"] = {"value": round(n * args.np_reps / nwall, 1), "unit": "pairs/s",
+                              "kernel_ms": round(e0.elapsed_time(e1) / args.np_reps, 4),
+                              "workload": "config2: 1M pairs (50% box-box, 30% box-sphere, 20% sphere-sphere)"}
+        nctx.close()
+    if not args.no_cpu:
+        procs = args.cpu_procs or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds, procs, 240)
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -189,9 +189,25 @@ int nans_download_state(nans_ctx *ctx, nans_bodies_soa *bodies);
 int nans_step_plain(nans_ctx *ctx, int32_t nsteps);
 
 /* Advance every world by nsteps with the batched path: one persistent launch per
- * step over all worlds (at most 512 bodies per world), the steps of the call
- * captured in a hipGraph that later calls with the same nsteps replay. */
+ * step over all worlds (at most 512 bodies per world).  The step (work-counter
+ * reset + step kernel) is captured once in a hipGraph and replayed nsteps times;
+ * the graph is kept until the batch, the parameters or the tuning change. */
 int nans_step_batched(nans_ctx *ctx, int32_t nsteps);
+
+/* Device time of the last nans_step_batched call, from HIP events recorded on
+ * the stream its kernels run on, and the launch shape it used. */
+typedef struct nans_timing {
+    int32_t steps;
+    float elapsed_ms;
+    int32_t threads_per_world;
+    int32_t workgroups;
+    int32_t lds_bytes;
+} nans_timing;
+int nans_batched_timing(nans_ctx *ctx, nans_timing *timing);
+
+/* Overwrite pos, vel, angvel, angles (and force, torque; NULL = zero) of the
+ * uploaded batch in place, keeping buffers and the captured graph. */
+int nans_set_state(nans_ctx *ctx, const nans_bodies_soa *bodies);
 
 /* Keep the contact list of each world's last step (off by default); at most
  * 16 n + 64 contacts per world of n bodies are kept. */

@@ -43,56 +43,9 @@ __device__ inline Shape load_shape(const NarrowIn &s, int64_t i, int64_t n)
     return sh;
 }
 
-// One lane per pair for GJK; the wave then runs EPA for each of its lanes whose
-// GJK found an intersection, one pair at a time, with the polytope in LDS.
-__global__ __launch_bounds__(256) void k_narrowphase(int64_t n, NarrowIn a, NarrowIn b, NarrowParams p,
-                                                     NarrowOutDev o, unsigned long long *overflow)
+__device__ inline void write_pair(const NarrowOutDev &o, int64_t i, int64_t n, int g, int status, V3 nrm, V3 pa,
+                                  V3 pb, double depth)
 {
-    __shared__ EpaLds lds[4];
-    EpaLds &L = lds[threadIdx.x >> 6];
-    const int lane = lane_id();
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const bool active = i < n;
-
-    Shape A, B;
-    int g = GJK_NO, calls = 0;
-    Simplex S;
-    S.n = 0;
-    if (active) {
-        A = load_shape(a, i, n);
-        B = load_shape(b, i, n);
-        g = gjk(A, B, p, S, &calls);
-    }
-    int status = EPA_NONE;
-    V3 pa = mk3(0.0f, 0.0f, 0.0f), pb = pa, nrm = pa;
-    double depth = 0.0;
-
-    uint64_t m = ballot(active && g == GJK_FOUND);
-    while (m) {
-        const int src = ffs64(m);
-        m &= m - 1;
-        if (lane == src) {
-            shape_store(L.sa, A);
-            shape_store(L.sb, B);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                st3(L.vp, k, S.s[k].P);
-                st3(L.va, k, S.s[k].A);
-                st3(L.vb, k, S.s[k].B);
-            }
-        }
-        wave_sync();
-        EpaResult r = epa_wave(L, p);
-        if (lane == src) {
-            status = r.status;
-            pa = r.pa;
-            pb = r.pb;
-            nrm = r.n;
-            depth = r.depth;
-        }
-        wave_sync();
-    }
-    if (!active) return;
     if (o.gjk) o.gjk[i] = g;
     if (o.contact) o.contact[i] = status == EPA_CONTACT ? 1 : 0;
     if (o.status) o.status[i] = status;
@@ -100,7 +53,43 @@ __global__ __launch_bounds__(256) void k_narrowphase(int64_t n, NarrowIn a, Narr
     if (o.pa) { o.pa[i] = pa.x; o.pa[n + i] = pa.y; o.pa[2 * n + i] = pa.z; }
     if (o.pb) { o.pb[i] = pb.x; o.pb[n + i] = pb.y; o.pb[2 * n + i] = pb.z; }
     if (o.depth) o.depth[i] = depth;
-    if (status == EPA_OVERFLOW) atomicAdd(overflow, 1ull);
+}
+
+// Candidates of one workgroup: pairs [base, base + count) of the input arrays.
+struct PairSrc {
+    NarrowIn a, b;
+    int64_t base, n;
+    NarrowOutDev o;
+    unsigned long long *overflow;
+    __device__ void shapes(int c, Shape &A, Shape &B) const
+    {
+        A = load_shape(a, base + c, n);
+        B = load_shape(b, base + c, n);
+    }
+    __device__ void gjk_result(int c, int g) const
+    {
+        const V3 z = mk3(0.0f, 0.0f, 0.0f);
+        write_pair(o, base + c, n, g, EPA_NONE, z, z, z, 0.0);
+    }
+    __device__ void epa_result(int c, const EpaResult &r) const
+    {
+        write_pair(o, base + c, n, GJK_FOUND, r.status, r.n, r.pa, r.pb, r.depth);
+        if (r.status == EPA_OVERFLOW) atomicAdd(overflow, 1ull);
+    }
+};
+
+constexpr int NP_RANGE = 1024;  // pairs per workgroup of the narrowphase entry point
+
+__global__ __launch_bounds__(256) void k_narrowphase(int64_t n, NarrowIn a, NarrowIn b, NarrowParams p,
+                                                     NarrowOutDev o, unsigned long long *overflow, int *qidx,
+                                                     SimplexRec *qsim)
+{
+    __shared__ EpaLds lds[4];
+    __shared__ NarrowShared sh;
+    const int64_t base = (int64_t)blockIdx.x * NP_RANGE;
+    const int count = (int)min((int64_t)NP_RANGE, n - base);
+    PairSrc src{a, b, base, n, o, overflow};
+    narrow_block<256>(src, count, p, lds, sh, qidx + base, qsim + base);
 }
 
 __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
@@ -175,6 +164,9 @@ int nans_create(nans_ctx **out, const nans_params *p)
     c->p = q;
     c->stream = nullptr;
     c->worlds = nullptr;
+    c->np_cap = 0;
+    c->np_qidx = nullptr;
+    c->np_qsim = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -199,6 +191,8 @@ int nans_destroy(nans_ctx *ctx)
     NANS_HIP(hipDeviceSynchronize());
     worlds_release(ctx);
     hipFree(ctx->d_counters);
+    if (ctx->np_qidx) hipFree(ctx->np_qidx);
+    if (ctx->np_qsim) hipFree(ctx->np_qsim);
     hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return NANS_OK;
@@ -235,11 +229,22 @@ int nans_narrowphase(nans_ctx *ctx, int64_t n, const nans_shapes_soa *a, const n
     k_fill_u64<<<1, 64, 0, s>>>(ctx->d_counters, 1, 0ull);
     NANS_HIP(hipGetLastError());
     if (n > 0) {
+        if (n > ctx->np_cap) {
+            hipFree(ctx->np_qidx);
+            hipFree(ctx->np_qsim);
+            ctx->np_qidx = nullptr;
+            ctx->np_qsim = nullptr;
+            ctx->np_cap = 0;
+            NANS_HIP(hipMalloc((void **)&ctx->np_qidx, sizeof(int) * (size_t)n));
+            NANS_HIP(hipMalloc((void **)&ctx->np_qsim, sizeof(SimplexRec) * (size_t)n));
+            ctx->np_cap = n;
+        }
         NarrowIn ia{a->type, a->pos, a->quat, a->half};
         NarrowIn ib{b->type, b->pos, b->quat, b->half};
         NarrowOutDev o{out->gjk, out->contact, out->status, out->normal, out->point_a, out->point_b, out->depth};
-        unsigned grid = (unsigned)((n + 255) / 256);
-        k_narrowphase<<<grid, 256, 0, s>>>(n, ia, ib, narrow_params(ctx->p), o, ctx->d_counters);
+        unsigned grid = (unsigned)((n + NP_RANGE - 1) / NP_RANGE);
+        k_narrowphase<<<grid, 256, 0, s>>>(n, ia, ib, narrow_params(ctx->p), o, ctx->d_counters, ctx->np_qidx,
+                                           (SimplexRec *)ctx->np_qsim);
         NANS_HIP(hipGetLastError());
     }
     unsigned long long ov = 0;

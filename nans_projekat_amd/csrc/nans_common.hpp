@@ -58,4 +58,7 @@ struct nans_ctx {
     hipStream_t own_stream;      // private stream for graph capture / replay
     unsigned long long *d_counters;  // small device scratch counters
     nans_worlds *worlds;
+    int64_t np_cap;                  // narrowphase entry point: queue capacity (pairs)
+    int *np_qidx;                    // queued GJK hits
+    void *np_qsim;                   // their simplices (SimplexRec)
 };

@@ -93,39 +93,55 @@ __device__ inline Vtx minkowski(const Shape &a, const Shape &b, V3 d)
 __device__ inline V3 triple_cross(V3 a, V3 b, V3 c) { return sub(mul(b, dot(c, a)), mul(a, dot(c, b))); }
 
 // ---------------------------------------------------------------------- GJK
-// Simplex kept in registers; erase() shifts like std::vector::erase.
+// Simplex in four named register slots (no array, so nothing is placed in
+// scratch); erase() shifts like std::vector::erase.
 struct Simplex {
-    Vtx s[4];
+    Vtx s0, s1, s2, s3;
     int n;
 };
 
-__device__ inline void simplex_erase(Simplex &S, int idx)
+// Per-component selects: a conditional on struct lvalues would select an address,
+// which keeps the simplex out of registers.
+__device__ __forceinline__ void fsel(V3 &d, bool c, const V3 &s)
 {
-    // idx is 0, 1 or 2; unrolled selects keep the array in registers
-    if (idx <= 0) S.s[0] = S.s[1];
-    if (idx <= 1) S.s[1] = S.s[2];
-    if (idx <= 2) S.s[2] = S.s[3];
+    d.x = c ? s.x : d.x;
+    d.y = c ? s.y : d.y;
+    d.z = c ? s.z : d.z;
+}
+
+__device__ __forceinline__ void vsel(Vtx &d, bool c, const Vtx &s)
+{
+    fsel(d.P, c, s.P);
+    fsel(d.A, c, s.A);
+    fsel(d.B, c, s.B);
+}
+
+__device__ __forceinline__ void simplex_erase(Simplex &S, int idx)
+{
+    vsel(S.s0, idx <= 0, S.s1);
+    vsel(S.s1, idx <= 1, S.s2);
+    vsel(S.s2, idx <= 2, S.s3);
     S.n -= 1;
 }
 
-__device__ inline void simplex_push(Simplex &S, const Vtx &v)
+__device__ __forceinline__ void simplex_push(Simplex &S, const Vtx &v)
 {
-    if (S.n == 0) S.s[0] = v;
-    else if (S.n == 1) S.s[1] = v;
-    else if (S.n == 2) S.s[2] = v;
-    else S.s[3] = v;
+    vsel(S.s0, S.n == 0, v);
+    vsel(S.s1, S.n == 1, v);
+    vsel(S.s2, S.n == 2, v);
+    vsel(S.s3, S.n >= 3, v);
     S.n += 1;
 }
 
 // EvolveSimplex (code/nans.cpp:571-769)
-__device__ inline int evolve_simplex(const Shape &a, const Shape &b, Simplex &S, int fp64)
+__device__ __forceinline__ int evolve_simplex(const Shape &a, const Shape &b, Simplex &S, int fp64)
 {
     V3 dir = normalize(sub(b.c, a.c));
     if (S.n == 1) {
         dir = mul(dir, -1.0f);
     } else if (S.n == 2) {
         // ClosestPointOnLine (code/nans.cpp:539-562)
-        V3 A = S.s[0].P, B = S.s[1].P;
+        V3 A = S.s0.P, B = S.s1.P;
         V3 seg = normalize(sub(B, A));
         float len = length(seg);
         float V = dot(neg(A), seg) / len;
@@ -141,9 +157,9 @@ __device__ inline int evolve_simplex(const Shape &a, const Shape &b, Simplex &S,
         }
         dir = neg(cp);
     } else if (S.n == 3) {
-        V3 n2o = neg(S.s[0].P);
-        V3 e1 = sub(S.s[1].P, S.s[0].P);
-        V3 e2 = sub(S.s[2].P, S.s[0].P);
+        V3 n2o = neg(S.s0.P);
+        V3 e1 = sub(S.s1.P, S.s0.P);
+        V3 e2 = sub(S.s2.P, S.s0.P);
         V3 tn = cross(e1, e2);
         V3 e1n = cross(e1, tn);
         V3 e2n = cross(tn, e2);
@@ -172,15 +188,15 @@ __device__ inline int evolve_simplex(const Shape &a, const Shape &b, Simplex &S,
             dir = tn;
         } else {
             dir = neg(tn);
-            Vtx t = S.s[1];
-            S.s[1] = S.s[2];
-            S.s[2] = t;
+            Vtx t = S.s1;
+            S.s1 = S.s2;
+            S.s2 = t;
         }
     } else if (S.n == 4) {
-        V3 DA = sub(S.s[0].P, S.s[3].P);
-        V3 DB = sub(S.s[1].P, S.s[3].P);
-        V3 DC = sub(S.s[2].P, S.s[3].P);
-        V3 D0 = mul(S.s[3].P, -1.0f);
+        V3 DA = sub(S.s0.P, S.s3.P);
+        V3 DB = sub(S.s1.P, S.s3.P);
+        V3 DC = sub(S.s2.P, S.s3.P);
+        V3 D0 = mul(S.s3.P, -1.0f);
         V3 abd = cross(DA, DB);
         V3 bcd = cross(DB, DC);
         V3 cad = cross(DC, DA);
@@ -330,6 +346,30 @@ __device__ inline double face_dist64(const EpaLds &L, int f)
     return d >= 0.0 ? d : -d;
 }
 
+// DPP move of a 32-bit value (GFX9 encodings: quad_perm 0x00-0xff, row_mirror
+// 0x140, row_half_mirror 0x141); every source lane of these patterns exists.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float readlane_f(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Minimum over the 64 lanes (no NaN inputs): quad and row reductions by DPP, then
+// the four row results through readlane.
+__device__ __forceinline__ float wave_min(float v)
+{
+    v = fminf(v, dpp_f<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = fminf(v, dpp_f<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = fminf(v, dpp_f<0x141>(v));  // row_half_mirror
+    v = fminf(v, dpp_f<0x140>(v));  // row_mirror
+    return fminf(fminf(readlane_f(v, 0), readlane_f(v, 16)), fminf(readlane_f(v, 32), readlane_f(v, 48)));
+}
+
 // Wave argmin over (value, index) among candidates; ties -> lowest index.
 template <typename T>
 __device__ inline void wave_argmin(bool &valid, T &v, int &i)
@@ -400,27 +440,30 @@ __device__ inline EpaResult epa_wave(EpaLds &L, const NarrowParams &p)
             r.status = EPA_EMPTY;
             break;
         }
-        // closest face (code/nans.cpp:807-822)
+        // closest face (code/nans.cpp:807-822): the scan starts from face 0's
+        // distance and takes strictly smaller ones, so the result is the first
+        // face holding the minimum of the distances below d0 (or face 0)
         int id0 = L.fid[0];
         float d0 = gabs(dot(ld3(L.fn, 0), ld3(L.vp, id0 & 255)));
-        bool valid = false;
-        float bv = 0.0f;
-        int bi = 0x7fffffff;
+        float key[EPA_FSLOTS];
+        float kmin = __builtin_inff();
 #pragma unroll
         for (int k = 0; k < EPA_FSLOTS; ++k) {
             int f = lane + 64 * k;
-            if (f < nf) {
-                float d = L.fd[f];
-                if (d < d0 && (!valid || d < bv)) {
-                    valid = true;
-                    bv = d;
-                    bi = f;
-                }
+            float d = f < nf ? L.fd[f] : __builtin_inff();
+            key[k] = d < d0 ? d : __builtin_inff();
+            kmin = fminf(kmin, key[k]);
+        }
+        const float m = wave_min(kmin);
+        int best = 0;
+        if (m != __builtin_inff()) {
+#pragma unroll
+            for (int k = EPA_FSLOTS - 1; k >= 0; --k) {
+                uint64_t b = ballot(key[k] == m);
+                if (b) best = 64 * k + ffs64(b);
             }
         }
-        wave_argmin(valid, bv, bi);
-        int best = valid ? bi : 0;
-        float cur = valid ? bv : d0;
+        float cur = m == __builtin_inff() ? d0 : L.fd[best];
         V3 N = ld3(L.fn, best);
         // new support along the stored normal
         Vtx ns;
@@ -592,6 +635,126 @@ __device__ inline EpaResult epa_wave(EpaLds &L, const NarrowParams &p)
         wave_sync();
     }
     return r;
+}
+
+// ---------------------------------------------------------------------- block driver
+// CheckCollision (code/nans.cpp:906-966) over the candidates [0, ncand) of one
+// workgroup, in two phases:
+//   GJK  every lane runs one candidate's EvolveSimplex loop; a lane whose loop
+//        ends takes the next candidate at once (work is handed out per wave by
+//        one LDS atomic), so a wave is not held by its slowest lane;
+//   EPA  the GJK hits are queued with their simplex; each wave takes queued hits
+//        one at a time (LDS atomic) and runs ResolveCollision on them with the
+//        polytope in its LDS block.
+// Results go through the source object, per candidate, so their order is that
+// of the candidates whatever the order of the work.
+struct NarrowShared {
+    int next;      // next candidate to hand out
+    int nepa;      // GJK hits queued
+    int epa_next;  // next queued hit to hand out
+};
+
+// A queued GJK hit's simplex: the supports on A and B of its four vertices; the
+// Minkowski point is recomputed as SupA - SupB, the expression that made it.
+struct SimplexRec {
+    float a[12], b[12];
+};
+
+__device__ __forceinline__ void simplex_rec_put(SimplexRec &q, const Vtx &v, int k)
+{
+    q.a[3 * k] = v.A.x; q.a[3 * k + 1] = v.A.y; q.a[3 * k + 2] = v.A.z;
+    q.b[3 * k] = v.B.x; q.b[3 * k + 1] = v.B.y; q.b[3 * k + 2] = v.B.z;
+}
+
+// Source interface (duck-typed):
+//   void shapes(int c, Shape &A, Shape &B) const       both shapes of candidate c
+//   void gjk_result(int c, int gjk) const               every candidate
+//   void epa_result(int c, const EpaResult &r) const    every GJK hit (lane 0 calls)
+template <int BLOCK, class Src>
+__device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, EpaLds *epa, NarrowShared &sh,
+                             int *qidx, SimplexRec *qsim)
+{
+    const int lane = lane_id();
+    if (threadIdx.x == 0) {
+        sh.next = 0;
+        sh.nepa = 0;
+        sh.epa_next = 0;
+    }
+    __syncthreads();
+    {
+        Shape A, B;
+        Simplex S;
+        S.n = 0;
+        int c = -1, calls = 0;
+        bool retired = false;
+        for (;;) {
+            const bool need = c < 0 && !retired;
+            const uint64_t m = ballot(need);
+            if (m) {
+                int base = 0;
+                if (lane == 0) base = atomicAdd(&sh.next, __popcll(m));
+                base = __builtin_amdgcn_readfirstlane(base);
+                if (need) {
+                    const int my = base + __popcll(m & ((1ull << lane) - 1ull));
+                    if (my < ncand) {
+                        c = my;
+                        src.shapes(c, A, B);
+                        S.n = 0;
+                        calls = 0;
+                    } else {
+                        retired = true;
+                    }
+                }
+            }
+            if (!ballot(c >= 0)) break;
+            if (c >= 0) {
+                const int res = evolve_simplex(A, B, S, np.fp64_islands);
+                ++calls;
+                if (res != GJK_STILL || calls > np.max_gjk_iterations) {
+                    const int g = res == GJK_STILL ? GJK_CAP : res;
+                    src.gjk_result(c, g);
+                    if (g == GJK_FOUND) {
+                        const int q = atomicAdd(&sh.nepa, 1);
+                        qidx[q] = c;
+                        simplex_rec_put(qsim[q], S.s0, 0);
+                        simplex_rec_put(qsim[q], S.s1, 1);
+                        simplex_rec_put(qsim[q], S.s2, 2);
+                        simplex_rec_put(qsim[q], S.s3, 3);
+                    }
+                    c = -1;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int nepa = sh.nepa;
+    EpaLds &L = epa[threadIdx.x >> 6];
+    for (;;) {
+        int job = 0;
+        if (lane == 0) job = atomicAdd(&sh.epa_next, 1);
+        job = __builtin_amdgcn_readfirstlane(job);
+        if (job >= nepa) break;
+        const int c = qidx[job];
+        Shape A, B;
+        src.shapes(c, A, B);
+        if (lane == 0) {
+            shape_store(L.sa, A);
+            shape_store(L.sb, B);
+        }
+        if (lane < 4) {
+            const SimplexRec &q = qsim[job];
+            V3 a = mk3(q.a[3 * lane], q.a[3 * lane + 1], q.a[3 * lane + 2]);
+            V3 b = mk3(q.b[3 * lane], q.b[3 * lane + 1], q.b[3 * lane + 2]);
+            st3(L.va, lane, a);
+            st3(L.vb, lane, b);
+            st3(L.vp, lane, sub(a, b));
+        }
+        wave_sync();
+        const EpaResult r = epa_wave(L, np);
+        if (lane == 0) src.epa_result(c, r);
+        wave_sync();
+    }
+    __syncthreads();
 }
 
 }  // namespace nans

@@ -110,66 +110,47 @@ __device__ inline void slot_sphere(const Planes &B, const float *brad, int slot,
     }
 }
 
-// GJK on one lane per candidate, EPA by the wave for each GJK hit; fills rec and
-// returns whether the candidate produced a contact.  All 64 lanes must call it.
-__device__ inline bool narrow_candidate(bool active, int64_t pair, int nb, int ns, const Planes &B,
-                                        const float *verts, const FloorDev &fl, const NarrowParams &np,
-                                        EpaLds &L, ContactRec &rec, unsigned long long *cnt)
-{
-    const int lane = lane_id();
-    Shape A, Bs;
-    Simplex S;
-    S.n = 0;
-    int g = GJK_NO, calls = 0;
-    PairId q{0, 0, 0};
-    int sa = 0, sb = 0;
-    if (active) {
-        q = decode_pair(pair, nb, ns);
+// Candidates of a world for the narrowphase driver: canonical pair indices
+// cand[0..ncand); results as a contact flag and record per candidate.
+struct WorldSrc {
+    Planes B;
+    const float *verts;
+    int nb, ns;
+    const FloorDev *fl;
+    const int *cand;
+    ContactRec *rec;
+    int *flag;
+    unsigned long long *cnt;
+    __device__ void shapes(int c, Shape &A, Shape &Bs) const
+    {
+        PairId q = decode_pair(cand[c], nb, ns);
+        int sa, sb;
         pair_slots(q, nb, sa, sb);
-        A = body_shape(B, verts, sa, nb, fl);
-        Bs = body_shape(B, verts, sb, nb, fl);
-        g = gjk(A, Bs, np, S, &calls);
+        A = body_shape(B, verts, sa, nb, *fl);
+        Bs = body_shape(B, verts, sb, nb, *fl);
     }
-    int status = EPA_NONE;
-    V3 pa = mk3(0.0f, 0.0f, 0.0f), pb = pa, nrm = pa;
-    uint64_t m = ballot(active && g == GJK_FOUND);
-    if (lane == 0 && m) atomicAdd(&cnt[CNT_GJK_FOUND], (unsigned long long)__popcll(m));
-    while (m) {
-        const int src = ffs64(m);
-        m &= m - 1;
-        if (lane == src) {
-            shape_store(L.sa, A);
-            shape_store(L.sb, Bs);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                st3(L.vp, k, S.s[k].P);
-                st3(L.va, k, S.s[k].A);
-                st3(L.vb, k, S.s[k].B);
-            }
-        }
-        wave_sync();
-        EpaResult r = epa_wave(L, np);
-        if (lane == src) {
-            status = r.status;
-            pa = r.pa;
-            pb = r.pb;
-            nrm = r.n;
-        }
-        wave_sync();
+    __device__ void gjk_result(int c, int g) const { flag[c] = 0; }
+    __device__ void epa_result(int c, const EpaResult &r) const
+    {
+        if (r.status == EPA_OVERFLOW) atomicAdd(&cnt[CNT_EPA_OVERFLOW], 1ull);
+        if (r.status != EPA_CONTACT) return;
+        PairId q = decode_pair(cand[c], nb, ns);
+        ContactRec &o = rec[c];
+        int sa, sb;
+        pair_slots(q, nb, sa, sb);
+        o.type = q.type;
+        o.ia = q.i;
+        o.ib = q.j;
+        o.slot_a = sa;
+        o.slot_b = sb;
+        o.pad0 = o.pad1 = o.pad2 = 0;
+        o.pa[0] = r.pa.x; o.pa[1] = r.pa.y; o.pa[2] = r.pa.z;
+        o.pb[0] = r.pb.x; o.pb[1] = r.pb.y; o.pb[2] = r.pb.z;
+        o.n[0] = r.n.x; o.n[1] = r.n.y; o.n[2] = r.n.z;
+        o.dln = o.dlt1 = o.dlt2 = 0.0f;
+        flag[c] = 1;
     }
-    if (status == EPA_OVERFLOW) atomicAdd(&cnt[CNT_EPA_OVERFLOW], 1ull);
-    rec.type = q.type;
-    rec.ia = q.i;
-    rec.ib = q.j;
-    rec.slot_a = sa;
-    rec.slot_b = sb;
-    rec.pad0 = rec.pad1 = rec.pad2 = 0;
-    rec.pa[0] = pa.x; rec.pa[1] = pa.y; rec.pa[2] = pa.z;
-    rec.pb[0] = pb.x; rec.pb[1] = pb.y; rec.pb[2] = pb.z;
-    rec.n[0] = nrm.x; rec.n[1] = nrm.y; rec.n[2] = nrm.z;
-    rec.dln = rec.dlt1 = rec.dlt2 = 0.0f;
-    return active && status == EPA_CONTACT;
-}
+};
 
 // Dependency rounds of SolveConstraints over con[0..ncon) (see the file comment).
 // head[] has room for n ints; done[] for ncon.
@@ -320,26 +301,24 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int *counts, int nb
     }
 }
 
+constexpr int NARROW_RANGE = 512;  // candidates per workgroup of the plain narrowphase
+
 __global__ __launch_bounds__(256) void k_plain_narrow(Planes B, const float *verts, int nb, int ns,
                                                       const FloorDev *flp, const int *cand, const int *ncand,
-                                                      NarrowParams np, ContactRec *rec, int *flag,
-                                                      unsigned long long *cnt)
+                                                      NarrowParams np, ContactRec *rec, int *flag, int *qidx,
+                                                      SimplexRec *qsim, unsigned long long *cnt)
 {
     __shared__ EpaLds lds[4];
+    __shared__ NarrowShared sh;
     __shared__ FloorDev fl;
     if (threadIdx.x == 0) fl = *flp;
     __syncthreads();
-    EpaLds &L = lds[threadIdx.x >> 6];
     const int total = *ncand;
-    for (int base = blockIdx.x * 256; base < total; base += gridDim.x * 256) {
-        int c = base + threadIdx.x;
-        bool active = c < total;
-        ContactRec r;
-        bool hit = narrow_candidate(active, active ? cand[c] : 0, nb, ns, B, verts, fl, np, L, r, cnt);
-        if (active) {
-            rec[c] = r;
-            flag[c] = hit ? 1 : 0;
-        }
+    for (int base = blockIdx.x * NARROW_RANGE; base < total; base += gridDim.x * NARROW_RANGE) {
+        const int count = min(NARROW_RANGE, total - base);
+        WorldSrc src{B, verts, nb, ns, &fl, cand + base, rec + base, flag + base, cnt};
+        narrow_block<256>(src, count, np, lds, sh, qidx + base, qsim + base);
+        if (threadIdx.x == 0) atomicAdd(&cnt[CNT_GJK_FOUND], (unsigned long long)sh.nepa);
     }
 }
 
@@ -409,8 +388,10 @@ struct BatchArgs {
     const FloorDev *fl;
     const float *wfric, *wrest;  // per-world coefficients (may be null)
     int *cand;              // per-workgroup scratch, cap entries each
-    ContactRec *con;
-    int *done;
+    ContactRec *rec;
+    int *flag;
+    int *qidx;
+    SimplexRec *qsim;
     int cap;
     ContactRec *rec_out;    // recording (may be null)
     const int64_t *rec_off;
@@ -434,12 +415,19 @@ __global__ __launch_bounds__(BLOCK) void k_batched_step(BatchArgs a)
     float *vertsL = bodyL + F_NLDS * nmax;             // [nmax][24]
     float *bradL = vertsL + 24 * nmax;                 // [nmax]
     int *headL = (int *)(bradL + nmax);                // [nmax]
-    EpaLds *epa = (EpaLds *)(headL + nmax + ((4 - (nmax & 3)) & 3));  // 16-byte aligned (nmax padded)
-    EpaLds &L = epa[tid >> 6];
-    int *cand = a.cand + (int64_t)blockIdx.x * a.cap;
-    ContactRec *con = a.con + (int64_t)blockIdx.x * a.cap;
-    int *done = a.done + (int64_t)blockIdx.x * a.cap;
-    const Planes G{a.planes, (int)a.N};
+    EpaLds *epa = (EpaLds *)(headL + nmax + ((4 - (nmax & 3)) & 3));
+    __shared__ NarrowShared nsh;
+    // per-workgroup scratch, a.cap entries each: candidates, their contact records
+    // and flags (the flags become the solver's done marks), the EPA queue (whose
+    // simplex memory then holds the ordered contact list)
+    const int64_t wg = (int64_t)blockIdx.x * a.cap;
+    int *cand = a.cand + wg;
+    ContactRec *rec = a.rec + wg;
+    int *flag = a.flag + wg;
+    int *done = flag;
+    int *qidx = a.qidx + wg;
+    SimplexRec *qsim = a.qsim + wg;
+    ContactRec *con = (ContactRec *)qsim;
 
     for (;;) {
         if (tid == 0) s_job = atomicAdd(a.queue, 1);
@@ -495,21 +483,21 @@ __global__ __launch_bounds__(BLOCK) void k_batched_step(BatchArgs a)
             atomicAdd(&a.cnt[CNT_CANDIDATES], (unsigned long long)ncand);
         }
         __syncthreads();
-        // narrowphase over the candidates, contacts appended in order
+        // narrowphase over the candidates (GJK per lane, EPA per wave)
+        {
+            WorldSrc src{Lb, vertsL, nb, ns, &fl, cand, rec, flag, a.cnt};
+            narrow_block<BLOCK>(src, ncand, a.k.np, epa, nsh, qidx, qsim);
+        }
+        if (tid == 0) atomicAdd(&a.cnt[CNT_GJK_FOUND], (unsigned long long)nsh.nepa);
+        // the contacts in candidate (= canonical) order; con reuses the queue's memory
         int ncon = 0;
         for (int base = 0; base < ncand; base += BLOCK) {
             int c = base + tid;
-            bool active = c < ncand;
-            ContactRec r;
-            bool hit = narrow_candidate(active, active ? cand[c] : 0, nb, ns, Lb, vertsL, fl, a.k.np, L, r, a.cnt);
+            bool hit = c < ncand && flag[c];
             int tot;
             int rk = block_rank<BLOCK>(hit, wave_tot, tot);
-            if (hit && ncon + rk < a.cap) con[ncon + rk] = r;
+            if (hit) con[ncon + rk] = rec[c];
             ncon += tot;
-        }
-        if (ncon > a.cap) {
-            if (tid == 0) atomicAdd(&a.cnt[CNT_CON_OVERFLOW], 1ull);
-            ncon = a.cap;
         }
         if (tid == 0) atomicAdd(&a.cnt[CNT_CONTACTS], (unsigned long long)ncon);
         __syncthreads();
@@ -571,6 +559,8 @@ struct nans_worlds {
     int *flag = nullptr;
     ContactRec *con = nullptr;
     int *done = nullptr;
+    int *qidx = nullptr;
+    SimplexRec *qsim = nullptr;
     int *blk_counts = nullptr;
     int *blk_offsets = nullptr;
     int *ncand = nullptr;
@@ -579,8 +569,10 @@ struct nans_worlds {
     int b_grid = 0;
     int b_cap = 0;
     int *b_cand = nullptr;
-    ContactRec *b_con = nullptr;
-    int *b_done = nullptr;
+    ContactRec *b_rec = nullptr;
+    int *b_flag = nullptr;
+    int *b_qidx = nullptr;
+    SimplexRec *b_qsim = nullptr;
     int *queue = nullptr;
     int queue_len = 0;
     // recording
@@ -596,7 +588,10 @@ struct nans_worlds {
     int graph_steps = -1;
     int graph_block = 0;
     nans_params graph_params;
-    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
+    int graph_grid = 0, graph_lds = 0;
+    int timed_steps = 0;
+    float timed_ms = 0.0f;
     int tuned_block = 0;   // autotuner's choice of threads per world (0: not tuned)
 };
 
@@ -610,11 +605,15 @@ static void wfree(void *p)
 static void release_batch_scratch(nans_worlds *W)
 {
     wfree(W->b_cand);
-    wfree(W->b_con);
-    wfree(W->b_done);
+    wfree(W->b_rec);
+    wfree(W->b_flag);
+    wfree(W->b_qidx);
+    wfree(W->b_qsim);
     W->b_cand = nullptr;
-    W->b_con = nullptr;
-    W->b_done = nullptr;
+    W->b_rec = nullptr;
+    W->b_flag = nullptr;
+    W->b_qidx = nullptr;
+    W->b_qsim = nullptr;
     W->b_grid = 0;
 }
 
@@ -645,10 +644,11 @@ int worlds_release(nans_ctx *ctx)
     for (void *p : {(void *)W->planes, (void *)W->verts, (void *)W->brad, (void *)W->offsets, (void *)W->nboxes,
                     (void *)W->order, (void *)W->wfric, (void *)W->wrest, (void *)W->floor, (void *)W->cnt,
                     (void *)W->cand, (void *)W->rec, (void *)W->flag, (void *)W->con, (void *)W->done,
+                    (void *)W->qidx, (void *)W->qsim,
                     (void *)W->blk_counts, (void *)W->blk_offsets, (void *)W->ncand, (void *)W->queue})
         wfree(p);
-    if (W->ev_in) hipEventDestroy(W->ev_in);
-    if (W->ev_out) hipEventDestroy(W->ev_out);
+    for (hipEvent_t ev : {W->ev_in, W->ev_out, W->ev_t0, W->ev_t1})
+        if (ev) hipEventDestroy(ev);
     release_batch_scratch(W);
     release_recording(W);
     delete W;
@@ -822,6 +822,8 @@ int nans_upload_worlds(nans_ctx *ctx, int32_t nworlds, const int32_t *world_offs
     NANS_TRY(dmalloc(&W->flag, (size_t)cap));
     NANS_TRY(dmalloc(&W->con, (size_t)cap));
     NANS_TRY(dmalloc(&W->done, (size_t)cap));
+    NANS_TRY(dmalloc(&W->qidx, (size_t)cap));
+    NANS_TRY(dmalloc(&W->qsim, (size_t)cap));
     NANS_TRY(dmalloc(&W->blk_counts, (size_t)W->max_blk));
     NANS_TRY(dmalloc(&W->blk_offsets, (size_t)W->max_blk));
     NANS_TRY(dmalloc(&W->ncand, 1));
@@ -974,9 +976,9 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
                 k_scan<<<1, SCAN_THREADS, 0, s>>>(W->blk_counts, nblk, W->blk_offsets, W->ncand, W->cand_cap, W->cnt);
                 k_plain_cull<true><<<nblk, 256, 0, s>>>(B, brad, nb, ns, npairs, W->floor, nullptr, W->blk_offsets,
                                                         W->cand, W->cand_cap, W->cnt);
-                int grid = std::max(1, std::min((W->cand_cap + 255) / 256, 2048));
+                int grid = std::max(1, std::min((W->cand_cap + NARROW_RANGE - 1) / NARROW_RANGE, 2048));
                 k_plain_narrow<<<grid, 256, 0, s>>>(B, verts, nb, ns, W->floor, W->cand, W->ncand, k.np, W->rec,
-                                                    W->flag, W->cnt);
+                                                    W->flag, W->qidx, W->qsim, W->cnt);
             } else {
                 k_zero_i32<<<1, 64, 0, s>>>(W->ncand, 1);
             }
@@ -1038,8 +1040,10 @@ static int batched_capture(nans_ctx *ctx, int nsteps, hipStream_t cs)
     if (grid > W->b_grid || cap != W->b_cap) {
         release_batch_scratch(W);
         NANS_TRY(dmalloc(&W->b_cand, (size_t)grid * cap));
-        NANS_TRY(dmalloc(&W->b_con, (size_t)grid * cap));
-        NANS_TRY(dmalloc(&W->b_done, (size_t)grid * cap));
+        NANS_TRY(dmalloc(&W->b_rec, (size_t)grid * cap));
+        NANS_TRY(dmalloc(&W->b_flag, (size_t)grid * cap));
+        NANS_TRY(dmalloc(&W->b_qidx, (size_t)grid * cap));
+        NANS_TRY(dmalloc(&W->b_qsim, (size_t)grid * cap));
         W->b_grid = grid;
         W->b_cap = cap;
     }
@@ -1062,21 +1066,23 @@ static int batched_capture(nans_ctx *ctx, int nsteps, hipStream_t cs)
     a.wfric = W->wfric;
     a.wrest = W->wrest;
     a.cand = W->b_cand;
-    a.con = W->b_con;
-    a.done = W->b_done;
+    a.rec = W->b_rec;
+    a.flag = W->b_flag;
+    a.qidx = W->b_qidx;
+    a.qsim = W->b_qsim;
     a.cap = W->b_cap;
     a.rec_out = W->record ? W->rec_buf : nullptr;
     a.rec_off = W->rec_off;
     a.rec_count = W->rec_count;
     a.rec_cap = W->rec_cap;
     a.cnt = W->cnt;
+    // one step: reset the work counter with a fill kernel (no memset node), then the
+    // persistent step kernel; a call of nsteps replays this graph nsteps times
     hipGraph_t g = nullptr;
+    a.queue = W->queue;
     NANS_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    k_zero_i32<<<(nsteps + 255) / 256, 256, 0, cs>>>(W->queue, nsteps);
-    for (int s = 0; s < nsteps; ++s) {
-        a.queue = W->queue + s;
-        k_batched_step<BLOCK><<<grid, BLOCK, lds, cs>>>(a);
-    }
+    k_zero_i32<<<1, 64, 0, cs>>>(W->queue, 1);
+    k_batched_step<BLOCK><<<grid, BLOCK, lds, cs>>>(a);
     hipError_t e = hipStreamEndCapture(cs, &g);
     NANS_REQUIRE(e == hipSuccess, NANS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
     drop_graph(W);
@@ -1086,6 +1092,8 @@ static int batched_capture(nans_ctx *ctx, int nsteps, hipStream_t cs)
     W->graph_steps = nsteps;
     W->graph_block = BLOCK;
     W->graph_params = ctx->p;
+    W->graph_grid = grid;
+    W->graph_lds = (int)lds;
     return NANS_OK;
 }
 
@@ -1099,24 +1107,29 @@ int batched_run(nans_ctx *ctx, int nsteps, int block)
     if (!W->ev_in) {
         NANS_HIP(hipEventCreateWithFlags(&W->ev_in, hipEventDisableTiming));
         NANS_HIP(hipEventCreateWithFlags(&W->ev_out, hipEventDisableTiming));
+        NANS_HIP(hipEventCreate(&W->ev_t0));
+        NANS_HIP(hipEventCreate(&W->ev_t1));
     }
-    bool reuse = W->graph && W->graph_steps == nsteps && W->graph_block == block &&
-                 memcmp(&W->graph_params, &ctx->p, sizeof(nans_params)) == 0;
+    bool reuse = W->graph && W->graph_block == block && memcmp(&W->graph_params, &ctx->p, sizeof(nans_params)) == 0;
     if (!reuse) {
         NANS_TRY(setup_floor(ctx, ctx->stream));
         NANS_HIP(hipStreamSynchronize(ctx->stream));
         switch (block) {
-        case 64: NANS_TRY(batched_capture<64>(ctx, nsteps, cs)); break;
-        case 128: NANS_TRY(batched_capture<128>(ctx, nsteps, cs)); break;
-        case 512: NANS_TRY(batched_capture<512>(ctx, nsteps, cs)); break;
-        default: NANS_TRY(batched_capture<256>(ctx, nsteps, cs)); break;
+        case 64: NANS_TRY(batched_capture<64>(ctx, 1, cs)); break;
+        case 128: NANS_TRY(batched_capture<128>(ctx, 1, cs)); break;
+        case 512: NANS_TRY(batched_capture<512>(ctx, 1, cs)); break;
+        default: NANS_TRY(batched_capture<256>(ctx, 1, cs)); break;
         }
     }
     NANS_HIP(hipEventRecord(W->ev_in, ctx->stream));
     NANS_HIP(hipStreamWaitEvent(cs, W->ev_in, 0));
-    NANS_HIP(hipGraphLaunch(W->graph, cs));
+    // events on the stream the kernels run on bracket the nsteps graph launches
+    NANS_HIP(hipEventRecord(W->ev_t0, cs));
+    for (int s = 0; s < nsteps; ++s) NANS_HIP(hipGraphLaunch(W->graph, cs));
+    NANS_HIP(hipEventRecord(W->ev_t1, cs));
     NANS_HIP(hipEventRecord(W->ev_out, cs));
     NANS_HIP(hipStreamWaitEvent(ctx->stream, W->ev_out, 0));
+    W->timed_steps = nsteps;
     return NANS_OK;
 }
 
@@ -1134,7 +1147,43 @@ int nans_step_batched(nans_ctx *ctx, int32_t nsteps)
     int block = W->tuned_block ? W->tuned_block : 256;
     NANS_TRY(batched_run(ctx, nsteps, block));
     NANS_HIP(hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    NANS_HIP(hipEventElapsedTime(&ms, W->ev_t0, W->ev_t1));
+    W->timed_ms = ms;
     return check_overflow(ctx);
+}
+
+int nans_batched_timing(nans_ctx *ctx, nans_timing *t)
+{
+    NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
+    NANS_REQUIRE(t, NANS_ERR_INVALID, "timing is NULL");
+    nans_worlds *W = ctx->worlds;
+    NANS_REQUIRE(W->timed_steps > 0, NANS_ERR_STATE, "no batched step has run");
+    t->steps = W->timed_steps;
+    t->elapsed_ms = W->timed_ms;
+    t->threads_per_world = W->graph_block;
+    t->workgroups = W->graph_grid;
+    t->lds_bytes = W->graph_lds;
+    return NANS_OK;
+}
+
+int nans_set_state(nans_ctx *ctx, const nans_bodies_soa *b)
+{
+    NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
+    NANS_REQUIRE(b && b->pos && b->vel && b->angvel && b->angles, NANS_ERR_INVALID, "a state array is NULL");
+    nans_worlds *W = ctx->worlds;
+    const int64_t N = W->N;
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    NANS_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<float> h((size_t)12 * N);
+    const float *src[4] = {b->pos, b->vel, b->angvel, b->angles};
+    for (int f = 0; f < 4; ++f) std::copy(src[f], src[f] + 3 * N, h.begin() + (size_t)3 * f * N);
+    NANS_HIP(hipMemcpy(W->planes, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+    std::vector<float> ft((size_t)6 * N, 0.0f);
+    if (b->force) std::copy(b->force, b->force + 3 * N, ft.begin());
+    if (b->torque) std::copy(b->torque, b->torque + 3 * N, ft.begin() + 3 * N);
+    NANS_HIP(hipMemcpy(W->planes + (size_t)F_FX * N, ft.data(), sizeof(float) * ft.size(), hipMemcpyHostToDevice));
+    return NANS_OK;
 }
 
 }  // extern "C"

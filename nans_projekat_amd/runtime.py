@@ -65,6 +65,11 @@ class Stats(ctypes.Structure):
                                               "capacity_overflow")]
 
 
+class Timing(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int32), ("elapsed_ms", ctypes.c_float), ("threads_per_world", ctypes.c_int32),
+                ("workgroups", ctypes.c_int32), ("lds_bytes", ctypes.c_int32)]
+
+
 # nans_contact (include/nans_gpu.h) as a numpy record
 CONTACT_DTYPE = np.dtype([("type", "<i4"), ("index_a", "<i4"), ("index_b", "<i4"), ("pad", "<i4"),
                           ("point_a", "<f4", 3), ("point_b", "<f4", 3), ("normal", "<f4", 3),
@@ -259,3 +264,17 @@ class Context:
 
     def reset_stats(self) -> None:
         check(self.lib.nans_reset_stats(self.h))
+
+    def set_state(self, scene: "_scenes.Scene") -> None:
+        """Overwrite the uploaded batch's state in place (same shape)."""
+        planes = lambda a: np.ascontiguousarray(np.asarray(a, np.float32).T)
+        st = {k: planes(getattr(scene, k)) for k in ("pos", "vel", "angvel", "angles", "force", "torque")}
+        self._state_keep = st
+        b = BodiesSoa(*[_np(st[k]) for k in ("pos", "vel", "angvel", "angles", "force", "torque")],
+                      None, None, None, None)
+        check(self.lib.nans_set_state(self.h, ctypes.byref(b)))
+
+    def batched_timing(self) -> dict:
+        t = Timing()
+        check(self.lib.nans_batched_timing(self.h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in Timing._fields_}

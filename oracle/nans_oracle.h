@@ -173,6 +173,14 @@ int ora_world_pairs_f32(int32_t nb, int32_t ns,
 void ora_box_vertices_f32(const float *pos, const float *angles, const float *scale, float *out);
 void ora_box_vertices_f64(const double *pos, const double *angles, const double *scale, double *out);
 
+/* Work counters of the step (see nans_oracle_impl.h): enable/disable (and reset),
+ * optionally applying the HIP step's bounding-sphere cull; read 32 counters. */
+#define ORA_NCOUNT 32
+void ora_counting_f32(int32_t enable, int32_t cull);
+void ora_counting_f64(int32_t enable, int32_t cull);
+void ora_get_counts_f32(int64_t *out);
+void ora_get_counts_f64(int64_t *out);
+
 /* RK4 of code/nans.cpp:50-78 on one body: v, w in/out [3]. */
 void ora_rk4_f32(float *v, float *w, const float *force, const float *torque,
                  float mass, float moi, const ora_step_params *p);

@@ -34,6 +34,37 @@ static inline float ora_cosf(float x) { return (float)cos((double)x); }
 #define COS cos
 #endif
 
+/* Work counters: events of the step (bodies integrated, pairs tested, GJK calls
+ * by simplex size, support evaluations, EPA iterations with their face and edge
+ * counts, contacts solved), from which the measurement harness derives the
+ * algorithmic flop count of a workload.  Off unless ora_counting_* enables them;
+ * the same call can switch on the product's bounding-sphere cull so that the
+ * counted work is the work the HIP step performs. */
+enum {
+    OC_BODY_BOX, OC_BODY_SPH, OC_PAIRS, OC_CAND, OC_EV0, OC_EV1, OC_EV2, OC_EV3, OC_EV4, OC_EVLEN,
+    OC_SUP_BOX, OC_SUP_SPH, OC_EPA_ITER, OC_EPA_F, OC_EPA_E, OC_EPA_FINAL, OC_CONSTRAINT, OC_SOLVER_IT,
+    OC_N
+};
+static long long SUF(g_counts)[32];
+static int SUF(g_counting);
+static int SUF(g_cull);
+#define OC(k, n)                                   \
+    do {                                           \
+        if (SUF(g_counting)) SUF(g_counts)[k] += (n); \
+    } while (0)
+
+void SUF(ora_counting)(int32_t enable, int32_t cull)
+{
+    SUF(g_counting) = enable;
+    SUF(g_cull) = cull;
+    memset(SUF(g_counts), 0, sizeof(SUF(g_counts)));
+}
+
+void SUF(ora_get_counts)(int64_t *out)
+{
+    for (int i = 0; i < 32; ++i) out[i] = SUF(g_counts)[i];
+}
+
 typedef struct { REAL x, y, z; } SUF(V3);
 #define V3 SUF(V3)
 typedef struct { V3 P, SupA, SupB; } SUF(Vert);  /* vertex, code/nans.h:245-255 */
@@ -192,6 +223,7 @@ static V3 SUF(sphere_support)(V3 c, REAL r, V3 d) { return vadd(c, vmul(vnormali
 
 static V3 SUF(shape_support)(const Shape *s, V3 d)
 {
+    OC(s->kind == ORA_SHAPE_BOX ? OC_SUP_BOX : OC_SUP_SPH, 1);
     return s->kind == ORA_SHAPE_BOX ? SUF(box_support)(s->verts, d) : SUF(sphere_support)(s->center, s->radius, d);
 }
 
@@ -236,6 +268,8 @@ static V3 SUF(triple_cross)(V3 A, V3 B, V3 C) { return vsub(vmul(B, vdot(C, A)),
 static int SUF(evolve_simplex)(const Shape *a, const Shape *b, Vert *S, int *n, int fp64)
 {
     V3 dir = vnormalize(vsub(b->center, a->center));
+    if (*n >= 0 && *n <= 4)
+        OC(OC_EV0 + *n, 1);
     switch (*n) {
     case 0:
         break;
@@ -320,6 +354,7 @@ static int SUF(evolve_simplex)(const Shape *a, const Shape *b, Vert *S, int *n, 
         return SUF(kNo);
     }
     /* termination test (nans.cpp:755-759); binary64 in the FP64-islands mode */
+    OC(OC_EVLEN, 1);
     if (fp64) {
         double dx = (double)dir.x, dy = (double)dir.y, dz = (double)dir.z;
         if (sqrt((dx * dx + dy * dy) + dz * dz) <= (double)0.0001f)
@@ -435,8 +470,10 @@ static int SUF(epa)(const Shape *a, const Shape *b, const Vert *S, const ora_np_
     int nverts = 4, made = 4, maxf = 4, maxe = 0, iters = 0;
     int result = ORA_EPA_ITER;
     int cur_iter = 0;
+    OC(OC_EPA_E, 4);
     while (cur_iter++ <= np->max_epa_iterations) {
         ++iters;
+        OC(OC_EPA_ITER, 1);
         if (p.nt == 0) {
             result = ORA_EPA_EMPTY;
             break;
@@ -495,8 +532,10 @@ static int SUF(epa)(const Shape *a, const Shape *b, const Vert *S, const ora_np_
                 out->depth = depth;
                 result = ORA_EPA_CONTACT;
             }
+            OC(OC_EPA_FINAL, 1);
             break;
         }
+        OC(OC_EPA_F, p.nt);
         for (int i = 0; i < p.nt;) {
             Tri *t = &p.t[i];
             V3 temp = vsub(ns.P, t->A.P);
@@ -519,6 +558,7 @@ static int SUF(epa)(const Shape *a, const Shape *b, const Vert *S, const ora_np_
             }
             ++i;
         }
+        OC(OC_EPA_E, p.ne);
         for (int i = 0; i < p.ne; ++i) {
             SUF(push_triangle)(&p, ns, p.e[i].A, p.e[i].B);
             ++made;
@@ -696,6 +736,8 @@ static V3 SUF(rk4)(int rot, REAL dt, V3 y0, V3 F, REAL m, const ora_step_params 
 static void SUF(integrate_forces)(World *w, const ora_step_params *p)
 {
     REAL dt = (REAL)p->dt;
+    OC(OC_BODY_BOX, w->nb);
+    OC(OC_BODY_SPH, w->ns);
     for (int i = 0; i < w->nb; ++i) {
         Body *b = &w->box[i];
         b->V = SUF(rk4)(0, dt, b->V, b->F, b->mass, p);
@@ -724,6 +766,34 @@ static void SUF(body_shape)(Shape *s, const Body *b, int kind)
     } else {
         s->radius = b->S.x;
     }
+}
+
+/* The HIP step's bounding-sphere cull (counting mode only): keep the pair unless
+ * the centre distance exceeds R_A + R_B plus a rounding margin (binary32 only). */
+static float SUF(shape_radius)(const Shape *s)
+{
+    if (s->kind != ORA_SHAPE_BOX)
+        return (float)s->radius;
+    float r2 = 0.0f;
+    for (int k = 0; k < 8; ++k) {
+        float dx = (float)s->verts[k].x - (float)s->center.x, dy = (float)s->verts[k].y - (float)s->center.y;
+        float dz = (float)s->verts[k].z - (float)s->center.z;
+        float d2 = (dx * dx + dy * dy) + dz * dz;
+        r2 = d2 > r2 ? d2 : r2;
+    }
+    return sqrtf(r2);
+}
+
+static int SUF(cull_keep)(const Shape *a, const Shape *b)
+{
+    float ra = SUF(shape_radius)(a), rb = SUF(shape_radius)(b);
+    float ax = (float)a->center.x, ay = (float)a->center.y, az = (float)a->center.z;
+    float bx = (float)b->center.x, by = (float)b->center.y, bz = (float)b->center.z;
+    float dx = bx - ax, dy = by - ay, dz = bz - az;
+    float dist2 = (dx * dx + dy * dy) + dz * dz;
+    float mag = fabsf(ax) + fabsf(ay) + fabsf(az) + fabsf(bx) + fabsf(by) + fabsf(bz);
+    float lim = (ra + rb) * 1.0001f + 1e-4f * (1.0f + mag + ra + rb);
+    return !(dist2 > lim * lim);
 }
 
 /* Body pair of a contact type: (A, kindA, B, kindB). */
@@ -756,6 +826,10 @@ static void SUF(test_pair)(World *w, ContactList *cl, int type, int ia, int ib, 
     Shape sa, sb;
     SUF(body_shape)(&sa, A, ka);
     SUF(body_shape)(&sb, B, kb);
+    OC(OC_PAIRS, 1);
+    if (SUF(g_cull) && !SUF(cull_keep)(&sa, &sb))
+        return;
+    OC(OC_CAND, 1);
     EpaOut o;
     int32_t st[ORA_NSTAT];
     int g = 0;
@@ -858,6 +932,8 @@ static void SUF(constraint)(World *w, Contact *pair, const ora_step_params *p)
 
     REAL sumN = 0, sumT1 = 0, sumT2 = 0, dln = 0, dlt1 = 0, dlt2 = 0;
     int iter = p->solver_iterations;
+    OC(OC_CONSTRAINT, 1);
+    OC(OC_SOLVER_IT, iter);
     while (iter--) {
         REAL LambdaN = (-JdVn + Bb) * JMJn;
         REAL old = sumN;

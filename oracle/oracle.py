@@ -191,3 +191,20 @@ def rk4(v, w, force, torque, mass, moi, params, precision: str = "f32"):
     prm = step_params(params)
     getattr(lib(), "ora_rk4_" + precision)(_ptr(v), _ptr(w), _ptr(f), _ptr(t), cty(mass), cty(moi), ctypes.byref(prm))
     return v, w
+
+
+COUNT_NAMES = ("body_box", "body_sphere", "pairs", "candidates", "evolve0", "evolve1", "evolve2", "evolve3",
+               "evolve4", "evolve_length", "support_box", "support_sphere", "epa_iter", "epa_faces", "epa_edges",
+               "epa_final", "constraints", "solver_iterations")
+
+
+def counting(enable: bool, cull: bool = False, precision: str = "f32") -> None:
+    """Switch the oracle's work counters on (reset) or off; cull applies the HIP
+    step's bounding-sphere cull so the counts describe the work it performs."""
+    getattr(lib(), "ora_counting_" + precision)(int(enable), int(cull))
+
+
+def counts(precision: str = "f32") -> dict:
+    out = np.zeros(32, np.int64)
+    getattr(lib(), "ora_get_counts_" + precision)(_ptr(out))
+    return {k: int(out[i]) for i, k in enumerate(COUNT_NAMES)}
