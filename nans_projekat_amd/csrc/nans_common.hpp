@@ -9,7 +9,7 @@
 #include <cstring>
 
 #include "../../include/nans_gpu.h"
-#include "nans_narrow.hpp"
+#include "nans_epa.hpp"
 
 namespace nans {
 

@@ -7,7 +7,7 @@
 // (batched path).
 #pragma once
 
-#include "nans_narrow.hpp"
+#include "nans_epa.hpp"
 
 namespace nans {
 
