@@ -28,6 +28,7 @@ import argparse
 import json
 import multiprocessing as mp
 import os
+import re
 import sys
 import time
 
@@ -61,6 +62,18 @@ FLOP_WEIGHTS = {
     "solver_iterations": 14, # one accumulate/clamp iteration
 }
 STATE_BYTES_PER_BODY_STEP = 164  # read P,V,W,A,S,m,I,F,T (92 B), write P,V,W,A and cleared F,T (72 B)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_batched_step_config3.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the batched step kernel: rocprofv3 FETCH_SIZE plus
+    WRITE_SIZE, each scaled by what the counter reads for a 1 GiB device copy
+    (the profile file records the commands and the factors)."""
+    if not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        m = re.search(r"^traffic_bytes_per_launch: (\d+)$", f.read(), re.M)
+    return int(m.group(1)) if m else None
 
 
 def algorithmic_flops(counts):
@@ -90,7 +103,7 @@ def cpu_baseline(sc, seconds, procs, nsteps):
     import numpy as np
 
     order = np.argsort(np.diff(sc.offsets))
-    sample = [int(order[i]) for i in np.linspace(0, len(order) - 1, 48).astype(int)]
+    sample = [int(order[i]) for i in np.linspace(0, len(order) - 1, 256).astype(int)]
     tasks = [(sc.world(w), nsteps) for w in sample]
     body_steps, busy = 0, 0.0
     with mp.get_context("spawn").Pool(procs, initializer=_cpu_init) as pool:
@@ -120,7 +133,20 @@ def main():
     ap.add_argument("--cpu-procs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--headline-only", action="store_true", help="only the batched config-3 measurement")
+    ap.add_argument("--large-steps", type=int, default=480, help="config-5 steps of the large-world leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device 3 times (scales rocprof FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": src.numel() * 4, "copies": 3}))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build runs no multi-GPU path (requested --gpus %d)" % args.gpus,
@@ -164,7 +190,7 @@ def main():
         tflops = flops / args.steps / (kernel_ms * 1e-3) / 1e12
         gbs = STATE_BYTES_PER_BODY_STEP * sc.nbodies / (kernel_ms * 1e-3) / 1e9
         roof = {"bound": "mfma", "achieved": round(tflops, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(tflops / PEAK_FP32_TFLOPS, 5), "traffic": None,
+                "frac": round(tflops / PEAK_FP32_TFLOPS, 5), "traffic": measured_traffic(),
                 "kernel": "k_batched_step", "compute": "vector FP32 (VALU); the kernels issue no MFMA, the "
                 "peak is MI355X's FP32 vector/matrix figure",
                 "flops_per_launch": round(flops / args.steps), "avg_launch_ms": round(kernel_ms, 5),
@@ -223,6 +249,23 @@ def main():
                               "kernel_ms": round(e0.elapsed_time(e1) / args.np_reps, 4),
                               "workload": "config2: 1M pairs (50% box-box, 30% box-sphere, 20% sphere-sphere)"}
         nctx.close()
+        # large single world (config 5) through the plain step
+        big = S.config5()
+        lctx = runtime.Context(big.params)
+        lctx.upload(big)
+        torch.cuda.synchronize()
+        l0 = time.perf_counter()
+        lctx.step_plain(args.large_steps)
+        torch.cuda.synchronize()
+        lwall = time.perf_counter() - l0
+        lst = lctx.stats()
+        out["large_world"] = {"value": round(big.nbodies * args.large_steps / lwall, 1), "unit": "body-steps/s",
+                              "ms_per_step": round(1e3 * lwall / args.large_steps, 4),
+                              "workload": "config5: 8192 boxes, dt 1/240, 20 iterations, %d steps, plain step"
+                                          % args.large_steps,
+                              "pair_tests_per_s": round(lst["pair_tests"] / lwall, 1),
+                              "contacts": lst["contacts"]}
+        lctx.close()
     if not args.no_cpu:
         procs = args.cpu_procs or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds, procs, 240)

@@ -1,0 +1,72 @@
+"""Config 5 (one world of 8192 boxes dropped on the ground box) through the plain
+step: a reduced block of the grid bit-exact with the oracle for 40 steps, the
+full world bit-exact after its first step (33.5M canonical pair tests), and the
+full 480-step run checked for finite state, no capacity overflow and boxes
+resting above the ground."""
+import numpy as np
+import pytest
+
+from nans_projekat_amd import runtime
+from nans_projekat_amd import scenes as S
+
+from golden_util import load_state_hash, state_hash
+
+pytestmark = pytest.mark.gpu
+
+
+def _state_equal(got, ws, where):
+    for f in ("pos", "vel", "angvel", "angles"):
+        np.testing.assert_array_equal(got[f], getattr(ws, f), err_msg="%s %s" % (where, f))
+
+
+def test_large_world_block_bit_exact(oracle_mod):
+    sc = S.config5(nx=12, ny=4, nz=12)
+    ctx = runtime.Context(sc.params)
+    ctx.upload(sc)
+    ctx.record_contacts(True)
+    ws = oracle_mod.WorldState(sc, 0, "f32")
+    ncontacts = 0
+    for k in range(4):
+        ctx.step_plain(40)
+        ref = ws.run(40, contacts=True)
+        _state_equal(ctx.download(), ws, "after %d steps" % (40 * (k + 1)))
+        got = ctx.contacts(0)
+        assert len(got) == len(ref)
+        np.testing.assert_array_equal(got["normal"], ref["normal"])
+        ncontacts = len(ref)
+    assert ncontacts > 0  # the boxes have landed
+    ctx.close()
+
+
+def test_full_large_world_first_step_bit_exact(oracle_mod):
+    sc = S.config5()
+    assert sc.nbodies == 8192
+    ctx = runtime.Context(sc.params)
+    ctx.upload(sc)
+    ctx.step_plain(1)
+    ws = oracle_mod.WorldState(sc, 0, "f32")
+    ws.run(1)
+    _state_equal(ctx.download(), ws, "step 1")
+    st = ctx.stats()
+    assert st["pair_tests"] == 8192 * 8191 // 2 + 8192
+    ctx.close()
+
+
+def test_full_large_world_rollout():
+    """The full 480-step run: finite state, no overflow, and the final state equal
+    bit for bit to the oracle's (its SHA-256 is frozen in tests/golden)."""
+    sc = S.config5()
+    ctx = runtime.Context(sc.params)
+    ctx.upload(sc)
+    ctx.step_plain(sc.nsteps)
+    got = ctx.download()
+    st = ctx.stats()
+    assert np.all(np.isfinite(got["pos"]))
+    assert st["epa_overflow"] == 0 and st["capacity_overflow"] == 0
+    assert st["contacts"] > 0
+    print("config 5, %d steps: %d pair tests, %d candidates, %d contacts" % (
+        sc.nsteps, st["pair_tests"], st["candidates"], st["contacts"]))
+    golden = load_state_hash("config5_480_steps")
+    assert golden["bodies"] == sc.nbodies and golden["steps"] == sc.nsteps
+    assert state_hash(got) == golden["sha256"]
+    ctx.close()

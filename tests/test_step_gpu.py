@@ -8,6 +8,8 @@ import pytest
 from nans_projekat_amd import runtime
 from nans_projekat_amd import scenes as S
 
+from golden_util import load_state_hash, state_hash
+
 pytestmark = pytest.mark.gpu
 
 STATE = ("pos", "vel", "angvel", "angles")
@@ -161,4 +163,8 @@ def test_full_config3_properties(oracle_mod):
         a, b = int(sc.offsets[w]), int(sc.offsets[w + 1])
         assert np.abs(got["pos"][a:b] - ws.pos).max() <= 1e-3
         _state_equal(got, ws, a, b, "world %d" % w)
+    # every world: the final state hashes to the oracle's (tests/golden/state_hashes.json)
+    golden = load_state_hash("config3_240_steps")
+    assert golden["bodies"] == sc.nbodies and golden["steps"] == sc.nsteps
+    assert state_hash(got) == golden["sha256"]
     ctx.close()
