@@ -220,6 +220,37 @@ int nans_get_contacts(nans_ctx *ctx, int32_t world, nans_contact *out, int32_t m
 int nans_get_stats(nans_ctx *ctx, nans_stats *stats);
 int nans_reset_stats(nans_ctx *ctx);
 
+/* ---- Differentiable rollout (no counterpart in the reference, which has no
+ * gradients; it differentiates the reference's physics block, code/nans.cpp:
+ * 1758-1762, through the batched step).
+ *
+ * nans_rollout advances every world by nsteps batched steps from the current
+ * state, as nans_step_batched does, and keeps the state (pos, vel, angvel,
+ * angles, force, torque) every checkpoint_every steps on the device.
+ *
+ * nans_adjoint returns the gradient of a loss L(final state) with respect to the
+ * initial state and the per-body mass and moment of inertia and the per-world
+ * friction and restitution, given dL/d(final state).  It is a hand-written
+ * reverse pass of RK4 (code/nans.cpp:50-78), of Constraint with its unrolled
+ * impulse iterations (code/nans.cpp:1020-1329) and of IntegrateVelocities
+ * (code/nans.cpp:1331-1349); the GJK/EPA outcome -- which pairs touch, in which
+ * order, with which normal and lever arms PointA - PosA, PointB - PosB -- is held
+ * fixed at the rollout's.  The segments between checkpoints are recomputed on the
+ * device.  The mass gradient counts mass only: when the moment of inertia is a
+ * function of the mass, add dl_moi * d(moi)/d(mass).  The batch is left at the
+ * rollout's final state.  Arrays are host memory: vectors [3][N] planes, dl_mass
+ * and dl_moi [N], dl_friction and dl_restitution [nworlds]; NULL inputs count as
+ * zero, NULL outputs are skipped.  At most 128 impulse iterations. */
+int nans_rollout(nans_ctx *ctx, int32_t nsteps, int32_t checkpoint_every);
+
+typedef struct nans_adjoint_io {
+    const float *dl_pos, *dl_vel, *dl_angvel, *dl_angles;  /* in: dL/d(final state) */
+    float *dl_pos0, *dl_vel0, *dl_angvel0, *dl_angles0;    /* out: dL/d(initial state) */
+    float *dl_mass, *dl_moi;                                /* out: per body */
+    float *dl_friction, *dl_restitution;                    /* out: per world */
+} nans_adjoint_io;
+int nans_adjoint(nans_ctx *ctx, const nans_adjoint_io *io);
+
 #ifdef __cplusplus
 }
 #endif

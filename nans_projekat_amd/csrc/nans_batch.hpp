@@ -1,0 +1,151 @@
+// nans_batch.hpp -- the device-resident world batch shared by the step
+// (nans_step.hip) and its adjoint (nans_adjoint.hip).
+#pragma once
+
+#include "nans_common.hpp"
+#include "nans_world.hpp"
+
+#include <vector>
+
+struct nans_worlds;
+
+namespace nans {
+
+constexpr int PLAIN_MAX_BODIES = 16384;  // solver head[] lives in LDS (64 KiB)
+constexpr int BATCH_MAX_BODIES = 512;    // bodies of one world kept in LDS by the batched path
+constexpr int SCAN_THREADS = 1024;
+constexpr int SOLVE_THREADS = 1024;
+constexpr int CULL_CHUNK = 4096;          // canonical pairs per block of the plain cull
+
+enum : int {
+    CNT_EPA_OVERFLOW = 0,
+    CNT_CAND_OVERFLOW,
+    CNT_CON_OVERFLOW,
+    CNT_CANDIDATES,
+    CNT_CONTACTS,
+    CNT_GJK_FOUND,
+    CNT_PAIR_TESTS,
+    CNT_N
+};
+
+// Arguments of k_batched_step (one persistent launch = one step of every world).
+struct BatchArgs {
+    float *planes;          // [F_NFIELDS][N]
+    int64_t N;
+    const int *offsets;     // [nw+1]
+    const int *nboxes;      // [nw]
+    const int *order;       // largest world first
+    int nworlds;
+    int nmax;               // LDS stride (>= largest world)
+    int *queue;             // this step's work counter
+    StepConsts k;
+    const FloorDev *fl;
+    const float *wfric, *wrest;  // per-world coefficients (may be null)
+    int *cand;              // per-workgroup scratch, cap entries each
+    ContactRec *rec;
+    int *flag;
+    int *qidx;
+    SimplexRec *qsim;
+    int cap;
+    ContactRec *rec_out;    // recording (may be null)
+    const int64_t *rec_off;
+    int *rec_count;
+    const int *rec_cap;
+    unsigned long long *cnt;
+};
+
+struct AdjointTape;  // nans_adjoint.hip
+void adjoint_release(nans_worlds *W);
+
+template <typename T>
+inline int dmalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    NANS_HIP(hipMalloc((void **)p, n * sizeof(T)));
+    return NANS_OK;
+}
+
+#define NANS_TRY(x)             \
+    do {                        \
+        int rc_ = (x);          \
+        if (rc_) return rc_;    \
+    } while (0)
+
+void wfree(void *p);
+StepConsts step_consts(const nans_params &p);
+int setup_floor(nans_ctx *ctx, hipStream_t s);
+int check_overflow(nans_ctx *ctx);
+// nsteps batched steps replayed from the captured one-step graph
+int batched_run(nans_ctx *ctx, int nsteps, int block);
+// threads per world of the batched step (the autotuner's choice or the default)
+int batched_block(nans_ctx *ctx);
+// One batched step launched directly on stream s, recording every world's
+// contact list into rec_out (per-world offsets rec_off, capacities rec_cap,
+// counts rec_count).
+int batched_step_recorded(nans_ctx *ctx, hipStream_t s, ContactRec *rec_out, const int64_t *rec_off,
+                          int *rec_count, const int *rec_cap);
+
+
+}  // namespace nans
+
+struct nans_worlds {
+    int nworlds = 0;
+    int64_t N = 0;
+    int nmax = 0;
+    int64_t max_pairs = 0;
+    std::vector<int> h_off, h_nb;
+    std::vector<float> h_fric, h_rest;
+    float *planes = nullptr;
+    float *verts = nullptr;
+    float *brad = nullptr;
+    int *offsets = nullptr;
+    int *nboxes = nullptr;
+    int *order = nullptr;
+    float *wfric = nullptr;
+    float *wrest = nullptr;
+    nans::FloorDev *floor = nullptr;
+    unsigned long long *cnt = nullptr;
+    // plain scratch
+    int cand_cap = 0;
+    int *cand = nullptr;
+    nans::ContactRec *rec = nullptr;
+    int *flag = nullptr;
+    nans::ContactRec *con = nullptr;
+    int *done = nullptr;
+    int *qidx = nullptr;
+    nans::SimplexRec *qsim = nullptr;
+    int *blk_counts = nullptr;
+    int *blk_offsets = nullptr;
+    int *ncand = nullptr;
+    int max_blk = 0;
+    // batched scratch
+    int b_grid = 0;
+    int b_cap = 0;
+    int *b_cand = nullptr;
+    nans::ContactRec *b_rec = nullptr;
+    int *b_flag = nullptr;
+    int *b_qidx = nullptr;
+    nans::SimplexRec *b_qsim = nullptr;
+    int *queue = nullptr;
+    int queue_len = 0;
+    // recording
+    int record = 0;
+    nans::ContactRec *rec_buf = nullptr;
+    int64_t *rec_off = nullptr;
+    int *rec_count = nullptr;
+    int *rec_cap = nullptr;
+    std::vector<int64_t> h_rec_off;
+    std::vector<int> h_rec_cap;
+    // graph cache of the batched step
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = -1;
+    int graph_block = 0;
+    nans_params graph_params;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
+    int graph_grid = 0, graph_lds = 0;
+    int timed_steps = 0;
+    float timed_ms = 0.0f;
+    int tuned_block = 0;   // autotuner's choice of threads per world (0: not tuned)
+    nans::AdjointTape *adj = nullptr;  // checkpoints and scratch of the differentiable rollout
+};

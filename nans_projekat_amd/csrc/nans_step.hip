@@ -21,8 +21,7 @@
 // dynamic bodies.  Contacts applied in one round touch disjoint bodies, and every
 // body sees its contacts in list order, so the result equals the serial loop bit
 // for bit.  The floor's velocities are never written, so it orders nothing.
-#include "nans_common.hpp"
-#include "nans_world.hpp"
+#include "nans_batch.hpp"
 
 #include <algorithm>
 #include <climits>
@@ -35,22 +34,6 @@ using namespace nans;
 
 namespace nans {
 
-constexpr int PLAIN_MAX_BODIES = 16384;  // solver head[] lives in LDS (64 KiB)
-constexpr int BATCH_MAX_BODIES = 512;    // bodies of one world kept in LDS by the batched path
-constexpr int SCAN_THREADS = 1024;
-constexpr int SOLVE_THREADS = 1024;
-constexpr int CULL_CHUNK = 4096;          // canonical pairs per block of the plain cull
-
-enum : int {
-    CNT_EPA_OVERFLOW = 0,
-    CNT_CAND_OVERFLOW,
-    CNT_CON_OVERFLOW,
-    CNT_CANDIDATES,
-    CNT_CONTACTS,
-    CNT_GJK_FOUND,
-    CNT_PAIR_TESTS,
-    CNT_N
-};
 
 // ------------------------------------------------------------------ small block helpers
 template <int BLOCK>
@@ -375,30 +358,6 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
 }
 
 // ------------------------------------------------------------------ batched kernel
-struct BatchArgs {
-    float *planes;          // [F_NFIELDS][N]
-    int64_t N;
-    const int *offsets;     // [nw+1]
-    const int *nboxes;      // [nw]
-    const int *order;       // largest world first
-    int nworlds;
-    int nmax;               // LDS stride (>= largest world)
-    int *queue;             // this step's work counter
-    StepConsts k;
-    const FloorDev *fl;
-    const float *wfric, *wrest;  // per-world coefficients (may be null)
-    int *cand;              // per-workgroup scratch, cap entries each
-    ContactRec *rec;
-    int *flag;
-    int *qidx;
-    SimplexRec *qsim;
-    int cap;
-    ContactRec *rec_out;    // recording (may be null)
-    const int64_t *rec_off;
-    int *rec_count;
-    const int *rec_cap;
-    unsigned long long *cnt;
-};
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_batched_step(BatchArgs a)
@@ -535,69 +494,10 @@ __global__ void k_zero_u64(unsigned long long *p, int n)
 }  // namespace nans
 
 // ====================================================================== host side
-struct nans_worlds {
-    int nworlds = 0;
-    int64_t N = 0;
-    int nmax = 0;
-    int64_t max_pairs = 0;
-    std::vector<int> h_off, h_nb;
-    std::vector<float> h_fric, h_rest;
-    float *planes = nullptr;
-    float *verts = nullptr;
-    float *brad = nullptr;
-    int *offsets = nullptr;
-    int *nboxes = nullptr;
-    int *order = nullptr;
-    float *wfric = nullptr;
-    float *wrest = nullptr;
-    FloorDev *floor = nullptr;
-    unsigned long long *cnt = nullptr;
-    // plain scratch
-    int cand_cap = 0;
-    int *cand = nullptr;
-    ContactRec *rec = nullptr;
-    int *flag = nullptr;
-    ContactRec *con = nullptr;
-    int *done = nullptr;
-    int *qidx = nullptr;
-    SimplexRec *qsim = nullptr;
-    int *blk_counts = nullptr;
-    int *blk_offsets = nullptr;
-    int *ncand = nullptr;
-    int max_blk = 0;
-    // batched scratch
-    int b_grid = 0;
-    int b_cap = 0;
-    int *b_cand = nullptr;
-    ContactRec *b_rec = nullptr;
-    int *b_flag = nullptr;
-    int *b_qidx = nullptr;
-    SimplexRec *b_qsim = nullptr;
-    int *queue = nullptr;
-    int queue_len = 0;
-    // recording
-    int record = 0;
-    ContactRec *rec_buf = nullptr;
-    int64_t *rec_off = nullptr;
-    int *rec_count = nullptr;
-    int *rec_cap = nullptr;
-    std::vector<int64_t> h_rec_off;
-    std::vector<int> h_rec_cap;
-    // graph cache of the batched step
-    hipGraphExec_t graph = nullptr;
-    int graph_steps = -1;
-    int graph_block = 0;
-    nans_params graph_params;
-    hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
-    int graph_grid = 0, graph_lds = 0;
-    int timed_steps = 0;
-    float timed_ms = 0.0f;
-    int tuned_block = 0;   // autotuner's choice of threads per world (0: not tuned)
-};
 
 namespace nans {
 
-static void wfree(void *p)
+void wfree(void *p)
 {
     if (p) hipFree(p);
 }
@@ -651,12 +551,13 @@ int worlds_release(nans_ctx *ctx)
         if (ev) hipEventDestroy(ev);
     release_batch_scratch(W);
     release_recording(W);
+    adjoint_release(W);
     delete W;
     ctx->worlds = nullptr;
     return NANS_OK;
 }
 
-static StepConsts step_consts(const nans_params &p)
+StepConsts step_consts(const nans_params &p)
 {
     StepConsts k;
     k.np = narrow_params(p);
@@ -670,7 +571,7 @@ static StepConsts step_consts(const nans_params &p)
     return k;
 }
 
-static int setup_floor(nans_ctx *ctx, hipStream_t s)
+int setup_floor(nans_ctx *ctx, hipStream_t s)
 {
     const nans_params &p = ctx->p;
     k_floor_setup<<<1, 64, 0, s>>>(ctx->worlds->floor, p.floor_pos[0], p.floor_pos[1], p.floor_pos[2],
@@ -679,20 +580,6 @@ static int setup_floor(nans_ctx *ctx, hipStream_t s)
     return NANS_OK;
 }
 
-template <typename T>
-static int dmalloc(T **p, size_t n)
-{
-    *p = nullptr;
-    if (n == 0) n = 1;
-    NANS_HIP(hipMalloc((void **)p, n * sizeof(T)));
-    return NANS_OK;
-}
-
-#define NANS_TRY(x)             \
-    do {                        \
-        int rc_ = (x);          \
-        if (rc_) return rc_;    \
-    } while (0)
 
 static int alloc_recording(nans_ctx *ctx)
 {
@@ -721,6 +608,19 @@ static size_t batch_lds_bytes(int nmax, int block)
     int pad = (4 - (nmax & 3)) & 3;
     return sizeof(float) * ((size_t)(F_NLDS + 24 + 1) * nmax) + sizeof(int) * (size_t)(nmax + pad) +
            sizeof(EpaLds) * (size_t)(block / 64);
+}
+
+}  // namespace nans
+
+namespace nans {
+
+int check_overflow(nans_ctx *ctx)
+{
+    unsigned long long c[CNT_N];
+    NANS_HIP(hipMemcpy(c, ctx->worlds->cnt, sizeof(c), hipMemcpyDeviceToHost));
+    NANS_REQUIRE(c[CNT_CAND_OVERFLOW] == 0 && c[CNT_CON_OVERFLOW] == 0, NANS_ERR_CAPACITY,
+                 "pair candidates or contacts exceeded the scratch capacity");
+    return NANS_OK;
 }
 
 }  // namespace nans
@@ -942,14 +842,6 @@ int nans_reset_stats(nans_ctx *ctx)
     return NANS_OK;
 }
 
-static int check_overflow(nans_ctx *ctx)
-{
-    unsigned long long c[CNT_N];
-    NANS_HIP(hipMemcpy(c, ctx->worlds->cnt, sizeof(c), hipMemcpyDeviceToHost));
-    NANS_REQUIRE(c[CNT_CAND_OVERFLOW] == 0 && c[CNT_CON_OVERFLOW] == 0, NANS_ERR_CAPACITY,
-                 "pair candidates or contacts exceeded the scratch capacity");
-    return NANS_OK;
-}
 
 int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
 {
@@ -1028,8 +920,9 @@ static int batched_config(nans_ctx *ctx, int *grid, size_t *lds)
     return NANS_OK;
 }
 
+// Launch shape, per-workgroup scratch and kernel arguments of the batched step.
 template <int BLOCK>
-static int batched_capture(nans_ctx *ctx, int nsteps, hipStream_t cs)
+static int batched_prepare(nans_ctx *ctx, BatchArgs &a, int *grid_out, size_t *lds_out)
 {
     nans_worlds *W = ctx->worlds;
     int grid = 0;
@@ -1047,13 +940,10 @@ static int batched_capture(nans_ctx *ctx, int nsteps, hipStream_t cs)
         W->b_grid = grid;
         W->b_cap = cap;
     }
-    if (nsteps > W->queue_len) {
-        wfree(W->queue);
-        W->queue = nullptr;
-        NANS_TRY(dmalloc(&W->queue, (size_t)nsteps));
-        W->queue_len = nsteps;
+    if (W->queue_len < 1) {
+        NANS_TRY(dmalloc(&W->queue, 1));
+        W->queue_len = 1;
     }
-    BatchArgs a;
     a.planes = W->planes;
     a.N = W->N;
     a.offsets = W->offsets;
@@ -1076,10 +966,23 @@ static int batched_capture(nans_ctx *ctx, int nsteps, hipStream_t cs)
     a.rec_count = W->rec_count;
     a.rec_cap = W->rec_cap;
     a.cnt = W->cnt;
+    a.queue = W->queue;
+    *grid_out = grid;
+    *lds_out = lds;
+    return NANS_OK;
+}
+
+template <int BLOCK>
+static int batched_capture(nans_ctx *ctx, int nsteps, hipStream_t cs)
+{
+    nans_worlds *W = ctx->worlds;
+    BatchArgs a;
+    int grid = 0;
+    size_t lds = 0;
+    NANS_TRY(batched_prepare<BLOCK>(ctx, a, &grid, &lds));
     // one step: reset the work counter with a fill kernel (no memset node), then the
     // persistent step kernel; a call of nsteps replays this graph nsteps times
     hipGraph_t g = nullptr;
-    a.queue = W->queue;
     NANS_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     k_zero_i32<<<1, 64, 0, cs>>>(W->queue, 1);
     k_batched_step<BLOCK><<<grid, BLOCK, lds, cs>>>(a);
@@ -1133,6 +1036,41 @@ int batched_run(nans_ctx *ctx, int nsteps, int block)
     return NANS_OK;
 }
 
+int batched_block(nans_ctx *ctx) { return ctx->worlds->tuned_block ? ctx->worlds->tuned_block : 256; }
+
+template <int BLOCK>
+static int batched_recorded_launch(nans_ctx *ctx, hipStream_t s, ContactRec *rec_out, const int64_t *rec_off,
+                                   int *rec_count, const int *rec_cap)
+{
+    BatchArgs a;
+    int grid = 0;
+    size_t lds = 0;
+    NANS_TRY(batched_prepare<BLOCK>(ctx, a, &grid, &lds));
+    a.rec_out = rec_out;
+    a.rec_off = rec_off;
+    a.rec_count = rec_count;
+    a.rec_cap = rec_cap;
+    k_zero_i32<<<1, 64, 0, s>>>(a.queue, 1);
+    k_batched_step<BLOCK><<<grid, BLOCK, lds, s>>>(a);
+    NANS_HIP(hipGetLastError());
+    return NANS_OK;
+}
+
+int batched_step_recorded(nans_ctx *ctx, hipStream_t s, ContactRec *rec_out, const int64_t *rec_off, int *rec_count,
+                          const int *rec_cap)
+{
+    nans_worlds *W = ctx->worlds;
+    NANS_REQUIRE(W->nmax <= BATCH_MAX_BODIES, NANS_ERR_CAPACITY,
+                 "the batched step keeps a world in LDS: at most %d bodies per world (largest: %d)",
+                 BATCH_MAX_BODIES, W->nmax);
+    switch (batched_block(ctx)) {
+    case 64: return batched_recorded_launch<64>(ctx, s, rec_out, rec_off, rec_count, rec_cap);
+    case 128: return batched_recorded_launch<128>(ctx, s, rec_out, rec_off, rec_count, rec_cap);
+    case 512: return batched_recorded_launch<512>(ctx, s, rec_out, rec_off, rec_count, rec_cap);
+    default: return batched_recorded_launch<256>(ctx, s, rec_out, rec_off, rec_count, rec_cap);
+    }
+}
+
 }  // namespace nans
 
 extern "C" {
@@ -1144,7 +1082,7 @@ int nans_step_batched(nans_ctx *ctx, int32_t nsteps)
     if (nsteps == 0) return NANS_OK;
     NANS_HIP(hipSetDevice(ctx->p.device));
     nans_worlds *W = ctx->worlds;
-    int block = W->tuned_block ? W->tuned_block : 256;
+    int block = batched_block(ctx);
     NANS_TRY(batched_run(ctx, nsteps, block));
     NANS_HIP(hipStreamSynchronize(ctx->stream));
     float ms = 0.0f;

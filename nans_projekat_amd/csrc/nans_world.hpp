@@ -7,6 +7,7 @@
 // (batched path).
 #pragma once
 
+#include "../../include/nans_gpu.h"
 #include "nans_epa.hpp"
 
 namespace nans {
@@ -24,9 +25,9 @@ enum : int {
 struct Planes {
     float *base;
     int stride;
-    __device__ float &at(int f, int i) const { return base[f * stride + i]; }
-    __device__ V3 get3(int f, int i) const { return mk3(at(f, i), at(f + 1, i), at(f + 2, i)); }
-    __device__ void set3(int f, int i, V3 v) const
+    __host__ __device__ float &at(int f, int i) const { return base[f * stride + i]; }
+    __host__ __device__ V3 get3(int f, int i) const { return mk3(at(f, i), at(f + 1, i), at(f + 2, i)); }
+    __host__ __device__ void set3(int f, int i, V3 v) const
     {
         at(f, i) = v.x;
         at(f + 1, i) = v.y;
@@ -50,7 +51,7 @@ struct FloorDev {
 
 // ------------------------------------------------------------------ RK4 (code/nans.cpp:50-78)
 // MovementFunction: (1/m)(F + m*g*(0,-1,0) - damping*v), with 1/m in binary64
-__device__ inline V3 movement(V3 v, V3 F, float m, const StepConsts &c)
+__host__ __device__ inline V3 movement(V3 v, V3 F, float m, const StepConsts &c)
 {
     float inv = (float)(1.0 / (double)m);
     V3 g = mul(mk3(0.0f, -1.0f, 0.0f), m * c.gravity);
@@ -58,14 +59,14 @@ __device__ inline V3 movement(V3 v, V3 F, float m, const StepConsts &c)
 }
 
 // RotationFunction: (1/I)(tau - damping*w)
-__device__ inline V3 rotation(V3 w, V3 T, float I, const StepConsts &c)
+__host__ __device__ inline V3 rotation(V3 w, V3 T, float I, const StepConsts &c)
 {
     float inv = 1.0f / I;
     return mul(sub(T, mul(w, c.damping)), inv);
 }
 
 template <bool ROT>
-__device__ inline V3 rk4(float dt, V3 y0, V3 F, float m, const StepConsts &c)
+__host__ __device__ inline V3 rk4(float dt, V3 y0, V3 F, float m, const StepConsts &c)
 {
     auto f = [&](V3 y) { return ROT ? rotation(y, F, m, c) : movement(y, F, m, c); };
     V3 k1 = mul(f(y0), dt);
@@ -78,7 +79,7 @@ __device__ inline V3 rk4(float dt, V3 y0, V3 F, float m, const StepConsts &c)
 
 // IntegrateForces for one body (code/nans.cpp:974-1018): RK4 on V and W with the
 // force and torque F, T (cleared by the caller).
-__device__ inline void integrate_forces_body(const Planes &B, int i, V3 F, V3 T, const StepConsts &c)
+__host__ __device__ inline void integrate_forces_body(const Planes &B, int i, V3 F, V3 T, const StepConsts &c)
 {
     V3 V = B.get3(F_VX, i), W = B.get3(F_WX, i);
     float m = B.at(F_MASS, i), I = B.at(F_MOI, i);
@@ -87,7 +88,7 @@ __device__ inline void integrate_forces_body(const Planes &B, int i, V3 F, V3 T,
 }
 
 // IntegrateVelocities for one body (code/nans.cpp:1331-1349)
-__device__ inline void integrate_velocities_body(const Planes &B, int i, float dt)
+__host__ __device__ inline void integrate_velocities_body(const Planes &B, int i, float dt)
 {
     B.set3(F_PX, i, add(B.get3(F_PX, i), mul(B.get3(F_VX, i), dt)));
     B.set3(F_AX, i, add(B.get3(F_AX, i), mul(B.get3(F_WX, i), dt)));
@@ -217,7 +218,7 @@ struct BodyRef {
     float mass, moi;
 };
 
-__device__ inline BodyRef body_ref(const Planes &B, int slot, const FloorDev &fl)
+__host__ __device__ inline BodyRef body_ref(const Planes &B, int slot, const FloorDev &fl)
 {
     BodyRef r;
     if (slot < 0) {
@@ -237,7 +238,7 @@ __device__ inline BodyRef body_ref(const Planes &B, int slot, const FloorDev &fl
 
 // Solve one contact against the current V, W of its bodies and apply the impulses.
 // friction / restitution are the pair's coefficients.
-__device__ inline void constraint(const Planes &B, ContactRec &c, const FloorDev &fl, const StepConsts &k,
+__host__ __device__ inline void constraint(const Planes &B, ContactRec &c, const FloorDev &fl, const StepConsts &k,
                                   float Cf, float Cr)
 {
     BodyRef A = body_ref(B, c.slot_a, fl);

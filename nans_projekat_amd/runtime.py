@@ -70,6 +70,12 @@ class Timing(ctypes.Structure):
                 ("workgroups", ctypes.c_int32), ("lds_bytes", ctypes.c_int32)]
 
 
+class AdjointIo(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("dl_pos", "dl_vel", "dl_angvel", "dl_angles", "dl_pos0", "dl_vel0",
+                                               "dl_angvel0", "dl_angles0", "dl_mass", "dl_moi", "dl_friction",
+                                               "dl_restitution")]
+
+
 # nans_contact (include/nans_gpu.h) as a numpy record
 CONTACT_DTYPE = np.dtype([("type", "<i4"), ("index_a", "<i4"), ("index_b", "<i4"), ("pad", "<i4"),
                           ("point_a", "<f4", 3), ("point_b", "<f4", 3), ("normal", "<f4", 3),
@@ -278,3 +284,28 @@ class Context:
         t = Timing()
         check(self.lib.nans_batched_timing(self.h, ctypes.byref(t)))
         return {k: getattr(t, k) for k, _ in Timing._fields_}
+
+    # ------------------------------------------------------------------ differentiable rollout
+    def rollout(self, nsteps: int, checkpoint_every: int = 16) -> None:
+        """nsteps batched steps from the current state, keeping the state every
+        checkpoint_every steps for adjoint()."""
+        self._sync_stream()
+        check(self.lib.nans_rollout(self.h, int(nsteps), int(checkpoint_every)))
+
+    def adjoint(self, dl_pos=None, dl_vel=None, dl_angvel=None, dl_angles=None) -> dict:
+        """Gradient of a loss of the final state through the last rollout(), given
+        dL/d(final pos, vel, angvel, angles) as [N][3] arrays (None = zero).
+        Returns pos0, vel0, angvel0, angles0 ([N][3]), mass, moi ([N]) and
+        friction, restitution ([nworlds]); contact sets and frames are held at the
+        rollout's (nans_adjoint in include/nans_gpu.h)."""
+        n, nw = self.nbodies, self.nworlds
+        planes = lambda a: None if a is None else np.ascontiguousarray(np.asarray(a, np.float32).reshape(n, 3).T)
+        ins = [planes(a) for a in (dl_pos, dl_vel, dl_angvel, dl_angles)]
+        outs = {k: np.zeros((3, n), np.float32) for k in ("pos0", "vel0", "angvel0", "angles0")}
+        outs.update(mass=np.zeros(n, np.float32), moi=np.zeros(n, np.float32), friction=np.zeros(nw, np.float32),
+                    restitution=np.zeros(nw, np.float32))
+        io = AdjointIo(*[_np(a) for a in ins], *[_np(outs[k]) for k in ("pos0", "vel0", "angvel0", "angles0", "mass",
+                                                                         "moi", "friction", "restitution")])
+        self._sync_stream()
+        check(self.lib.nans_adjoint(self.h, ctypes.byref(io)))
+        return {k: (np.ascontiguousarray(v.T) if v.ndim == 2 else v) for k, v in outs.items()}

@@ -187,6 +187,50 @@ void ora_rk4_f32(float *v, float *w, const float *force, const float *torque,
 void ora_rk4_f64(double *v, double *w, const double *force, const double *torque,
                  double mass, double moi, const ora_step_params *p);
 
+/* ---- Frozen-manifold rollouts: the function the step's adjoint differentiates.
+ *
+ * The adjoint holds the GJK/EPA outcome of every step fixed: which pairs are in
+ * contact, in which order, and each contact's frame -- the unit normal N and the
+ * lever arms R1 = PointA - PosA, R2 = PointB - PosB that Constraint derives from
+ * the EPA points (nans.cpp:1115-1124).  Everything else of the step is a smooth
+ * (piecewise, through the impulse clamps) function of the state and parameters:
+ * RK4 (nans.cpp:50-78), Constraint with Depth = dot((PosA + R1) - (PosB + R2), N)
+ * (nans.cpp:1139) and the impulse iterations, IntegrateVelocities.
+ *
+ * ora_frozen_record_f32 runs the binary32 step (as ora_world_run_f32) and writes
+ * the frame of every contact of every step: out[] in step order, step_count[s]
+ * contacts in step s.  Slots number the world's bodies boxes first (0..nb-1),
+ * then spheres (nb..nb+ns-1); the floor is -1.  Returns 0, -1 on bad arguments,
+ * -2 when more than max_out frames occurred (*total holds the count needed).
+ *
+ * ora_frozen_run_* replays nsteps steps with those frames instead of the
+ * narrowphase.  With the recorded binary32 inputs, ora_frozen_run_f32 reproduces
+ * ora_world_run_f32 bit for bit; ora_frozen_run_f64 is the same function in
+ * binary64, on which finite differences of the adjoint are taken.  friction and
+ * restitution are arguments (in binary64, so that they can be perturbed finely);
+ * the binary32 replay rounds them to binary32 as the step does. */
+typedef struct ora_manifold {
+    int32_t type, slot_a, slot_b, pad;
+    float r1[3], r2[3], n[3];
+} ora_manifold;
+
+int ora_frozen_record_f32(int32_t nb, int32_t ns,
+                          float *pos, float *vel, float *angvel, float *angles,
+                          float *force, float *torque,
+                          const float *scale, const float *mass, const float *moi,
+                          const ora_step_params *p, int32_t nsteps,
+                          ora_manifold *out, int64_t max_out, int32_t *step_count, int64_t *total);
+int ora_frozen_run_f32(int32_t nb, int32_t ns,
+                       float *pos, float *vel, float *angvel, float *angles,
+                       float *force, float *torque, const float *mass, const float *moi,
+                       const ora_step_params *p, double friction, double restitution, int32_t nsteps,
+                       const ora_manifold *m, const int32_t *step_count);
+int ora_frozen_run_f64(int32_t nb, int32_t ns,
+                       double *pos, double *vel, double *angvel, double *angles,
+                       double *force, double *torque, const double *mass, const double *moi,
+                       const ora_step_params *p, double friction, double restitution, int32_t nsteps,
+                       const ora_manifold *m, const int32_t *step_count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -882,20 +882,48 @@ static void SUF(detect_collisions)(World *w, ContactList *cl, const ora_step_par
 }
 
 /* Constraint (nans.cpp:1020-1329) on a copy of the contact */
-static void SUF(constraint)(World *w, Contact *pair, const ora_step_params *p)
+/* The contact frame of Constraint (nans.cpp:1115-1124): unit normal (with the
+ * zero-normal fallback) and lever arms from the bodies' positions. */
+static void SUF(contact_frame)(World *w, const Contact *pair, V3 *N, V3 *R1, V3 *R2)
 {
-    V3 N = vnormalize(pair->N);
     Body *A, *B;
     int ka, kb;
     SUF(pair_bodies)(w, pair->type, pair->ia, pair->ib, &A, &ka, &B, &kb);
+    *N = vnormalize(pair->N);
+    if (veq(*N, mk3(0, 0, 0)))
+        *N = vnormalize(vsub(B->P, A->P)); /* nans.cpp:1115-1119 */
+    *R1 = vsub(pair->PA, A->P);
+    *R2 = vsub(pair->PB, B->P);
+}
+
+/* Constraint's solve and apply (nans.cpp:1125-1329) for a contact frame; A and
+ * B's velocities are updated (B's only when two_sided).  dl[3] receives the
+ * applied impulses. */
+static void SUF(constraint_solve)(Body *A, Body *B, int two_sided, V3 N, V3 R1, V3 R2,
+                                  const ora_step_params *p, REAL Cf, REAL Cr, REAL *dl);
+
+static void SUF(constraint)(World *w, Contact *pair, const ora_step_params *p)
+{
+    Body *A, *B;
+    int ka, kb;
+    SUF(pair_bodies)(w, pair->type, pair->ia, pair->ib, &A, &ka, &B, &kb);
+    V3 N, R1, R2;
+    SUF(contact_frame)(w, pair, &N, &R1, &R2);
+    int two_sided = pair->type == ORA_CC || pair->type == ORA_CS || pair->type == ORA_SS;
+    REAL dl[3];
+    SUF(constraint_solve)(A, B, two_sided, N, R1, R2, p, (REAL)p->friction, (REAL)p->restitution, dl);
+    pair->dln = dl[0];
+    pair->dlt1 = dl[1];
+    pair->dlt2 = dl[2];
+}
+
+static void SUF(constraint_solve)(Body *A, Body *B, int two_sided, V3 N, V3 R1, V3 R2,
+                                  const ora_step_params *p, REAL Cf, REAL Cr, REAL *dl)
+{
     V3 PosA = A->P, PosB = B->P;
     REAL InvI1 = (REAL)1 / A->moi, InvI2 = (REAL)1 / B->moi;
     REAL InvM1 = (REAL)1 / A->mass, InvM2 = (REAL)1 / B->mass;
     V3 V1 = A->V, V2 = B->V, W1 = A->W, W2 = B->W;
-    if (veq(N, mk3(0, 0, 0)))
-        N = vnormalize(vsub(PosB, PosA)); /* nans.cpp:1115-1119 */
-    V3 R1 = vsub(pair->PA, PosA);
-    V3 R2 = vsub(pair->PB, PosB);
     V3 T1;
     if (N.x >= (REAL)0.57735f)
         T1 = mk3(N.y, -N.x, 0);
@@ -912,9 +940,7 @@ static void SUF(constraint)(World *w, Contact *pair, const ora_step_params *p)
     V3 dVn = vsub(vsub(vadd(V1, vcross(W1, N)), V2), vcross(W2, N));
     REAL JdVn = vdot(dVn, N);
     REAL Beta = (REAL)p->beta;
-    REAL Cr = (REAL)p->restitution;
     REAL Bb = -Beta / (REAL)p->dt * Depth + Cr * JdVn;
-    REAL Cf = (REAL)p->friction;
     V3 R1T1 = vcross(R1, T1);
     V3 R2T1 = vcross(R2, T1);
     V3 R1T2 = vcross(R1, T2);
@@ -966,9 +992,9 @@ static void SUF(constraint)(World *w, Contact *pair, const ora_step_params *p)
         if (sumT2 > MaxLT2) sumT2 = MaxLT2;
         dlt2 = sumT2 - oldT2;
     }
-    pair->dln = dln;
-    pair->dlt1 = dlt1;
-    pair->dlt2 = dlt2;
+    dl[0] = dln;
+    dl[1] = dlt1;
+    dl[2] = dlt2;
     V3 LinearImpulse = vmul(N, dln);
     V3 LinearImpulseT1 = vmul(T1, dlt1);
     V3 LinearImpulseT2 = vmul(T2, dlt2);
@@ -978,7 +1004,6 @@ static void SUF(constraint)(World *w, Contact *pair, const ora_step_params *p)
     V3 AngularI2T1 = vmul(R2T1, dlt1);
     V3 AngularI1T2 = vmul(R1T2, dlt2);
     V3 AngularI2T2 = vmul(R2T2, dlt2);
-    int two_sided = pair->type == ORA_CC || pair->type == ORA_CS || pair->type == ORA_SS;
     A->V = vadd(A->V, vmul(LinearImpulse, InvM1));
     if (two_sided) B->V = vsub(B->V, vmul(LinearImpulse, InvM2));
     A->W = vadd(A->W, vmul(AngularI1, InvI1));
@@ -1132,6 +1157,139 @@ int SUF(ora_world_run)(int32_t nb, int32_t ns,
     free(w.box);
     free(w.sph);
     return 0;
+}
+
+/* ---------------------------------------------------------------- frozen-manifold rollouts */
+static Body *SUF(slot_body)(World *w, int slot)
+{
+    if (slot < 0) return &w->floor;
+    return slot < w->nb ? &w->box[slot] : &w->sph[slot - w->nb];
+}
+
+#if ORA_F32
+int SUF(ora_frozen_record)(int32_t nb, int32_t ns,
+                           REAL *pos, REAL *vel, REAL *angvel, REAL *angles,
+                           REAL *force, REAL *torque,
+                           const REAL *scale, const REAL *mass, const REAL *moi,
+                           const ora_step_params *p, int32_t nsteps,
+                           ora_manifold *out, int64_t max_out, int32_t *step_count, int64_t *total)
+{
+    if (nb < 0 || ns < 0 || nsteps < 0 || !SUF(params_ok)(p) || !pos || !vel || !angvel || !angles ||
+        !scale || !mass || !moi || !step_count || max_out < 0 || (max_out > 0 && !out))
+        return -1;
+    World w;
+    SUF(world_load)(&w, nb, ns, pos, vel, angvel, angles, force, torque, scale, mass, moi, p);
+    ContactList cl;
+    memset(&cl, 0, sizeof(cl));
+    int64_t k = 0;
+    for (int s = 0; s < nsteps; ++s) {
+        cl.n = 0;
+        SUF(integrate_forces)(&w, p);
+        SUF(detect_collisions)(&w, &cl, p);
+        /* frames from the positions the solve reads (they do not change during it) */
+        for (int c = 0; c < cl.n; ++c, ++k) {
+            if (k >= max_out) continue;
+            const Contact *ct = &cl.c[c];
+            V3 N, R1, R2;
+            SUF(contact_frame)(&w, ct, &N, &R1, &R2);
+            ora_manifold *m = &out[k];
+            m->type = ct->type;
+            switch (ct->type) {
+            case ORA_CC: m->slot_a = ct->ia; m->slot_b = ct->ib; break;
+            case ORA_CS: m->slot_a = ct->ia; m->slot_b = nb + ct->ib; break;
+            case ORA_CF: m->slot_a = ct->ia; m->slot_b = -1; break;
+            case ORA_SS: m->slot_a = nb + ct->ia; m->slot_b = nb + ct->ib; break;
+            default: m->slot_a = nb + ct->ia; m->slot_b = -1; break;
+            }
+            m->pad = 0;
+            m->r1[0] = R1.x; m->r1[1] = R1.y; m->r1[2] = R1.z;
+            m->r2[0] = R2.x; m->r2[1] = R2.y; m->r2[2] = R2.z;
+            m->n[0] = N.x; m->n[1] = N.y; m->n[2] = N.z;
+        }
+        step_count[s] = cl.n;
+        for (int c = 0; c < cl.n; ++c)
+            SUF(constraint)(&w, &cl.c[c], p);
+        SUF(integrate_velocities)(&w, p);
+        SUF(rebuild_models)(&w);
+    }
+    if (total) *total = k;
+    SUF(world_store)(&w, pos, vel, angvel, angles, force, torque);
+    free(cl.c);
+    free(w.box);
+    free(w.sph);
+    return k > max_out ? -2 : 0;
+}
+#endif
+
+int SUF(ora_frozen_run)(int32_t nb, int32_t ns,
+                        REAL *pos, REAL *vel, REAL *angvel, REAL *angles,
+                        REAL *force, REAL *torque, const REAL *mass, const REAL *moi,
+                        const ora_step_params *p, double friction, double restitution, int32_t nsteps,
+                        const ora_manifold *m, const int32_t *step_count)
+{
+    if (nb < 0 || ns < 0 || nsteps < 0 || !SUF(params_ok)(p) || !pos || !vel || !angvel || !angles ||
+        !mass || !moi || (nsteps > 0 && !step_count))
+        return -1;
+    World w;
+    memset(&w, 0, sizeof(w));
+    w.nb = nb;
+    w.ns = ns;
+    w.box = (Body *)calloc((size_t)(nb > 0 ? nb : 1), sizeof(Body));
+    w.sph = (Body *)calloc((size_t)(ns > 0 ? ns : 1), sizeof(Body));
+    for (int k = 0; k < nb + ns; ++k) {
+        Body *b = SUF(slot_body)(&w, k);
+        b->P = mk3(pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]);
+        b->V = mk3(vel[3 * k], vel[3 * k + 1], vel[3 * k + 2]);
+        b->W = mk3(angvel[3 * k], angvel[3 * k + 1], angvel[3 * k + 2]);
+        b->A = mk3(angles[3 * k], angles[3 * k + 1], angles[3 * k + 2]);
+        b->F = force ? mk3(force[3 * k], force[3 * k + 1], force[3 * k + 2]) : mk3(0, 0, 0);
+        b->T = torque ? mk3(torque[3 * k], torque[3 * k + 1], torque[3 * k + 2]) : mk3(0, 0, 0);
+        b->mass = mass[k];
+        b->moi = moi[k];
+    }
+    w.floor.P = mk3((REAL)p->floor_pos[0], (REAL)p->floor_pos[1], (REAL)p->floor_pos[2]);
+    w.floor.mass = (REAL)p->floor_mass;
+    w.floor.moi = (REAL)p->floor_moi;
+    const REAL dt = (REAL)p->dt;
+    int64_t k = 0;
+    int rc = 0;
+    for (int s = 0; s < nsteps && rc == 0; ++s) {
+        for (int i = 0; i < nb + ns; ++i) {  /* IntegrateForces without the corner refresh */
+            Body *b = SUF(slot_body)(&w, i);
+            b->V = SUF(rk4)(0, dt, b->V, b->F, b->mass, p);
+            b->W = SUF(rk4)(1, dt, b->W, b->T, b->moi, p);
+            b->F = mk3(0, 0, 0);
+            b->T = mk3(0, 0, 0);
+        }
+        for (int c = 0; c < step_count[s]; ++c, ++k) {
+            const ora_manifold *f = &m[k];
+            if (f->slot_a < 0 || f->slot_a >= nb + ns || f->slot_b < -1 || f->slot_b >= nb + ns) {
+                rc = -1;
+                break;
+            }
+            REAL dl[3];
+            SUF(constraint_solve)(SUF(slot_body)(&w, f->slot_a), SUF(slot_body)(&w, f->slot_b), f->slot_b >= 0,
+                                  mk3(f->n[0], f->n[1], f->n[2]), mk3(f->r1[0], f->r1[1], f->r1[2]),
+                                  mk3(f->r2[0], f->r2[1], f->r2[2]), p, (REAL)friction, (REAL)restitution, dl);
+        }
+        for (int i = 0; i < nb + ns; ++i) {  /* IntegrateVelocities */
+            Body *b = SUF(slot_body)(&w, i);
+            b->P = vadd(b->P, vmul(b->V, dt));
+            b->A = vadd(b->A, vmul(b->W, dt));
+        }
+    }
+    for (int k2 = 0; k2 < nb + ns; ++k2) {
+        const Body *b = SUF(slot_body)(&w, k2);
+        pos[3 * k2] = b->P.x; pos[3 * k2 + 1] = b->P.y; pos[3 * k2 + 2] = b->P.z;
+        vel[3 * k2] = b->V.x; vel[3 * k2 + 1] = b->V.y; vel[3 * k2 + 2] = b->V.z;
+        angvel[3 * k2] = b->W.x; angvel[3 * k2 + 1] = b->W.y; angvel[3 * k2 + 2] = b->W.z;
+        angles[3 * k2] = b->A.x; angles[3 * k2 + 1] = b->A.y; angles[3 * k2 + 2] = b->A.z;
+        if (force) { force[3 * k2] = b->F.x; force[3 * k2 + 1] = b->F.y; force[3 * k2 + 2] = b->F.z; }
+        if (torque) { torque[3 * k2] = b->T.x; torque[3 * k2 + 1] = b->T.y; torque[3 * k2 + 2] = b->T.z; }
+    }
+    free(w.box);
+    free(w.sph);
+    return rc;
 }
 
 #if ORA_F32

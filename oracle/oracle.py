@@ -133,7 +133,7 @@ class WorldState:
         self.precision = precision
         self.nb = int(scene.nboxes[w])
         self.ns = (b - a) - self.nb
-        cp = lambda x: np.ascontiguousarray(x[a:b], dtype=ft)
+        cp = lambda x: np.array(x[a:b], dtype=ft, order="C")  # a copy: run() updates it in place
         self.pos, self.vel, self.angvel, self.angles = cp(scene.pos), cp(scene.vel), cp(scene.angvel), cp(scene.angles)
         self.force, self.torque = cp(scene.force), cp(scene.torque)
         self.scale, self.mass, self.moi = cp(scene.scale), cp(scene.mass), cp(scene.moi)
@@ -191,6 +191,60 @@ def rk4(v, w, force, torque, mass, moi, params, precision: str = "f32"):
     prm = step_params(params)
     getattr(lib(), "ora_rk4_" + precision)(_ptr(v), _ptr(w), _ptr(f), _ptr(t), cty(mass), cty(moi), ctypes.byref(prm))
     return v, w
+
+
+MANIFOLD_DTYPE = np.dtype([("type", "<i4"), ("slot_a", "<i4"), ("slot_b", "<i4"), ("pad", "<i4"),
+                           ("r1", "<f4", 3), ("r2", "<f4", 3), ("n", "<f4", 3)])
+
+
+def frozen_record(scene, w: int, nsteps: int):
+    """Binary32 rollout of world w recording every step's contact frames
+    (ora_frozen_record_f32).  Returns (frames, step_count, final WorldState)."""
+    ws = WorldState(scene, w, "f32")
+    prm = step_params(ws.params, ws.friction, ws.restitution)
+    counts_ = np.zeros(max(nsteps, 1), np.int32)
+    total = ctypes.c_int64(0)
+    cap = 64 * (ws.nb + ws.ns + 4) * max(nsteps, 1)
+    for _ in range(2):
+        frames = np.zeros(cap, MANIFOLD_DTYPE)
+        trial = WorldState(scene, w, "f32")
+        rc = lib().ora_frozen_record_f32(trial.nb, trial.ns, _ptr(trial.pos), _ptr(trial.vel), _ptr(trial.angvel),
+                                         _ptr(trial.angles), _ptr(trial.force), _ptr(trial.torque),
+                                         _ptr(trial.scale), _ptr(trial.mass), _ptr(trial.moi), ctypes.byref(prm),
+                                         int(nsteps), _ptr(frames), ctypes.c_int64(cap), _ptr(counts_),
+                                         ctypes.byref(total))
+        if rc == 0:
+            return frames[: total.value].copy(), counts_[:nsteps].copy(), trial
+        if rc != -2:
+            raise ValueError("oracle frozen_record rejected its arguments")
+        cap = int(total.value)
+    raise RuntimeError("frame capacity")
+
+
+def frozen_run(scene, w: int, frames, step_count, nsteps: int, precision: str = "f64", pos=None, vel=None,
+               angvel=None, angles=None, mass=None, moi=None, friction=None, restitution=None) -> dict:
+    """Replay world w for nsteps with recorded contact frames (ora_frozen_run_*),
+    from the scene's initial state or the given overrides; returns the final state."""
+    ft = np.float32 if precision == "f32" else np.float64
+    ws = WorldState(scene, w, precision)
+    pick = lambda x, d: np.array(d if x is None else x, dtype=ft).reshape(np.shape(d)).copy()
+    st = dict(pos=pick(pos, ws.pos), vel=pick(vel, ws.vel), angvel=pick(angvel, ws.angvel),
+              angles=pick(angles, ws.angles), force=ws.force.copy(), torque=ws.torque.copy())
+    m = pick(mass, ws.mass)
+    I = pick(moi, ws.moi)
+    prm = step_params(ws.params)
+    cf = float(ws.params.friction if ws.friction is None else ws.friction) if friction is None else float(friction)
+    cr = (float(ws.params.restitution if ws.restitution is None else ws.restitution) if restitution is None
+          else float(restitution))
+    fr = np.ascontiguousarray(frames, dtype=MANIFOLD_DTYPE)
+    sc = np.ascontiguousarray(step_count, dtype=np.int32)
+    fn = getattr(lib(), "ora_frozen_run_" + precision)
+    rc = fn(ws.nb, ws.ns, _ptr(st["pos"]), _ptr(st["vel"]), _ptr(st["angvel"]), _ptr(st["angles"]),
+            _ptr(st["force"]), _ptr(st["torque"]), _ptr(m), _ptr(I), ctypes.byref(prm), ctypes.c_double(cf),
+            ctypes.c_double(cr), int(nsteps), _ptr(fr), _ptr(sc))
+    if rc != 0:
+        raise ValueError("oracle frozen_run rejected its arguments")
+    return st
 
 
 COUNT_NAMES = ("body_box", "body_sphere", "pairs", "candidates", "evolve0", "evolve1", "evolve2", "evolve3",

@@ -56,3 +56,5 @@ def test_bad_arguments_return_codes():
     assert lib.nans_narrowphase(None, ctypes.c_int64(1), None, None, None, None) == runtime.ERR_INVALID
     assert lib.nans_set_params(None, None) == runtime.ERR_INVALID
     assert lib.nans_destroy(None) == runtime.OK
+    assert lib.nans_rollout(None, 240, 16) == runtime.ERR_STATE
+    assert lib.nans_adjoint(None, None) == runtime.ERR_STATE
