@@ -14,6 +14,9 @@ Also reported on the same line:
                   path) on a size-stratified subset of the same worlds, and the
                   batched/plain ratio;
   narrowphase     config 2 (1M box/sphere pairs, GJK + EPA) pairs/s;
+  large_world     config 5 (one world of 8192 boxes) through the plain step;
+  differentiable  config 4 (1024 x 16 cubes, 240 steps): checkpointed rollout
+                  plus the hand-written adjoint, body-steps/s;
   roofline        the batched step kernel: algorithmic FP32 flops per launch
                   (event counts frozen in tests/golden/work_counts.json times the
                   per-event weights below) over its average launch time, timed
@@ -130,6 +133,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--np-reps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--tune-steps", type=int, default=5,
+                    help="steps timed per workgroup size by the autotuner before the run (0: default size)")
     ap.add_argument("--cpu-procs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--headline-only", action="store_true", help="only the batched config-3 measurement")
@@ -165,6 +170,9 @@ def main():
                                                                     time.perf_counter() - t_gen), file=sys.stderr)
     ctx = runtime.Context(sc.params)
     ctx.upload(sc)
+    tuning = ctx.autotune(args.tune_steps) if args.tune_steps > 0 else None
+    if tuning:
+        print("autotune: %s" % tuning, file=sys.stderr)
     ctx.step_batched(max(1, args.warmup))
     ctx.set_state(sc)
     ctx.reset_stats()
@@ -206,7 +214,8 @@ def main():
                                   "dt 1/120, RK4, 10 impulse iterations, batched step",
                       "worlds": sc.nworlds, "bodies": sc.nbodies, "parallelism": "none (1 GPU)",
                       "threads_per_world": timing["threads_per_world"], "workgroups": timing["workgroups"],
-                      "lds_bytes": timing["lds_bytes"]},
+                      "lds_bytes": timing["lds_bytes"],
+                      "autotune_ms_per_step": tuning["ms_per_step"] if tuning else None},
            "roofline": roof,
            "step_counts": {k: stats[k] for k in ("pair_tests", "candidates", "gjk_found", "contacts")},
            "tolerance": "bit-exact with the binary32 CPU restatement of the reference step (tests/)"}
@@ -266,6 +275,30 @@ def main():
                               "pair_tests_per_s": round(lst["pair_tests"] / lwall, 1),
                               "contacts": lst["contacts"]}
         lctx.close()
+        # differentiable rollout (config 4): checkpointed forward + hand-written adjoint
+        dif = S.config4()
+        dctx = runtime.Context(dif.params)
+        dctx.upload(dif)
+        dl = np.full((dif.nbodies, 3), 1e-3, np.float32)
+        dctx.rollout(8, 4)
+        dctx.adjoint(dl_pos=dl)  # warm-up: graph capture, scratch allocation
+        dctx.set_state(dif)
+        torch.cuda.synchronize()
+        d0 = time.perf_counter()
+        dctx.rollout(dif.nsteps, 16)
+        torch.cuda.synchronize()
+        d1 = time.perf_counter()
+        dctx.adjoint(dl_pos=dl)
+        torch.cuda.synchronize()
+        d2 = time.perf_counter()
+        bsteps = dif.nbodies * dif.nsteps
+        out["differentiable"] = {"value": round(bsteps / (d2 - d0), 1), "unit": "body-steps/s (forward + adjoint)",
+                                 "forward_ms_per_step": round(1e3 * (d1 - d0) / dif.nsteps, 4),
+                                 "adjoint_ms_per_step": round(1e3 * (d2 - d1) / dif.nsteps, 4),
+                                 "workload": "config4: 1024 worlds x 16 cubes, %d steps, state checkpointed every "
+                                             "16 steps; gradients w.r.t. mass, moi, friction, restitution and the "
+                                             "initial state" % dif.nsteps}
+        dctx.close()
     if not args.no_cpu:
         procs = args.cpu_procs or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds, procs, 240)

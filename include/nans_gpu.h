@@ -209,6 +209,24 @@ int nans_batched_timing(nans_ctx *ctx, nans_timing *timing);
  * uploaded batch in place, keeping buffers and the captured graph. */
 int nans_set_state(nans_ctx *ctx, const nans_bodies_soa *bodies);
 
+/* Autotuner of the batched step (no counterpart in the reference, which steps one
+ * scene on one thread).  Each workgroup of the batched step runs one world: its
+ * size sets how many canonical pairs are culled and GJK-tested at once, how many
+ * EPA polytopes are in flight (one per wave) and the width of the solver's
+ * dependency rounds.  nans_autotune_batched times nsteps steps of the uploaded
+ * batch with 64, 128, 256 and 512 threads per world (sizes whose LDS does not fit
+ * are skipped, ms_per_step = -1), restores the state, force/torque and statistics,
+ * and keeps the fastest size for later nans_step_batched / nans_rollout calls.
+ * Results do not depend on the size.  nans_set_batched_block forces a size
+ * (0: the default, 256). */
+typedef struct nans_tuning {
+    int32_t threads_per_world;   /* chosen */
+    int32_t steps;               /* steps timed per candidate */
+    float ms_per_step[4];        /* 64, 128, 256, 512 threads per world */
+} nans_tuning;
+int nans_autotune_batched(nans_ctx *ctx, int32_t nsteps, nans_tuning *out);
+int nans_set_batched_block(nans_ctx *ctx, int32_t threads_per_world);
+
 /* Keep the contact list of each world's last step (off by default); at most
  * 16 n + 64 contacts per world of n bodies are kept. */
 int nans_record_contacts(nans_ctx *ctx, int32_t enable);

@@ -1105,6 +1105,81 @@ int nans_batched_timing(nans_ctx *ctx, nans_timing *t)
     return NANS_OK;
 }
 
+int nans_set_batched_block(nans_ctx *ctx, int32_t threads_per_world)
+{
+    NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
+    NANS_REQUIRE(threads_per_world == 0 || threads_per_world == 64 || threads_per_world == 128 ||
+                     threads_per_world == 256 || threads_per_world == 512,
+                 NANS_ERR_INVALID, "threads_per_world must be 0, 64, 128, 256 or 512 (got %d)", threads_per_world);
+    ctx->worlds->tuned_block = threads_per_world;
+    return NANS_OK;
+}
+
+int nans_autotune_batched(nans_ctx *ctx, int32_t nsteps, nans_tuning *out)
+{
+    NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
+    NANS_REQUIRE(nsteps >= 1, NANS_ERR_INVALID, "nsteps must be >= 1");
+    nans_worlds *W = ctx->worlds;
+    NANS_REQUIRE(W->nmax <= BATCH_MAX_BODIES, NANS_ERR_CAPACITY,
+                 "the batched step keeps a world in LDS: at most %d bodies per world (largest: %d)",
+                 BATCH_MAX_BODIES, W->nmax);
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    hipStream_t s = ctx->stream;
+    NANS_HIP(hipStreamSynchronize(s));
+    const size_t N = (size_t)W->N;
+    // the trials must leave no trace: keep state, force/torque and counters
+    float *keep = nullptr;
+    NANS_TRY(dmalloc(&keep, 18 * N));
+    unsigned long long cnt[CNT_N];
+    NANS_HIP(hipMemcpy(cnt, W->cnt, sizeof(cnt), hipMemcpyDeviceToHost));
+    NANS_HIP(hipMemcpyAsync(keep, W->planes, sizeof(float) * 12 * N, hipMemcpyDeviceToDevice, s));
+    NANS_HIP(hipMemcpyAsync(keep + 12 * N, W->planes + (size_t)F_FX * N, sizeof(float) * 6 * N,
+                            hipMemcpyDeviceToDevice, s));
+    const int cand[4] = {64, 128, 256, 512};
+    int best = 0;
+    float best_ms = 0.0f;
+    int rc = NANS_OK;
+    int max_lds = 0;
+    NANS_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->p.device));
+    for (int c = 0; c < 4 && rc == NANS_OK; ++c) {
+        float ms = -1.0f;
+        if (batch_lds_bytes(W->nmax, cand[c]) > (size_t)max_lds) {  // does not fit: skipped
+            if (out) out->ms_per_step[c] = ms;
+            continue;
+        }
+        // one untimed step captures the graph; then nsteps timed steps
+        int r = batched_run(ctx, 1, cand[c]);
+        if (r == NANS_OK) r = batched_run(ctx, nsteps, cand[c]);
+        if (r == NANS_OK) {
+            NANS_HIP(hipStreamSynchronize(s));
+            NANS_HIP(hipEventElapsedTime(&ms, W->ev_t0, W->ev_t1));
+            ms /= (float)nsteps;
+            if (!best || ms < best_ms) {
+                best = cand[c];
+                best_ms = ms;
+            }
+        } else {
+            rc = r;
+        }
+        if (out) out->ms_per_step[c] = ms;
+        NANS_HIP(hipMemcpyAsync(W->planes, keep, sizeof(float) * 12 * N, hipMemcpyDeviceToDevice, s));
+        NANS_HIP(hipMemcpyAsync(W->planes + (size_t)F_FX * N, keep + 12 * N, sizeof(float) * 6 * N,
+                                hipMemcpyDeviceToDevice, s));
+    }
+    NANS_HIP(hipStreamSynchronize(s));
+    wfree(keep);
+    NANS_HIP(hipMemcpy(W->cnt, cnt, sizeof(cnt), hipMemcpyHostToDevice));
+    if (rc != NANS_OK) return rc;
+    NANS_REQUIRE(best > 0, NANS_ERR_CAPACITY, "no workgroup size fits a world of %d bodies", W->nmax);
+    W->tuned_block = best;
+    W->timed_steps = 0;
+    if (out) {
+        out->threads_per_world = best;
+        out->steps = nsteps;
+    }
+    return NANS_OK;
+}
+
 int nans_set_state(nans_ctx *ctx, const nans_bodies_soa *b)
 {
     NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");

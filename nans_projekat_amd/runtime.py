@@ -70,6 +70,10 @@ class Timing(ctypes.Structure):
                 ("workgroups", ctypes.c_int32), ("lds_bytes", ctypes.c_int32)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("threads_per_world", ctypes.c_int32), ("steps", ctypes.c_int32), ("ms_per_step", ctypes.c_float * 4)]
+
+
 class AdjointIo(ctypes.Structure):
     _fields_ = [(k, ctypes.c_void_p) for k in ("dl_pos", "dl_vel", "dl_angvel", "dl_angles", "dl_pos0", "dl_vel0",
                                                "dl_angvel0", "dl_angles0", "dl_mass", "dl_moi", "dl_friction",
@@ -279,6 +283,18 @@ class Context:
         b = BodiesSoa(*[_np(st[k]) for k in ("pos", "vel", "angvel", "angles", "force", "torque")],
                       None, None, None, None)
         check(self.lib.nans_set_state(self.h, ctypes.byref(b)))
+
+    def autotune(self, nsteps: int = 5) -> dict:
+        """Time the batched step at 64/128/256/512 threads per world on the uploaded
+        batch and keep the fastest (nans_autotune_batched); the state is restored."""
+        self._sync_stream()
+        t = Tuning()
+        check(self.lib.nans_autotune_batched(self.h, int(nsteps), ctypes.byref(t)))
+        return {"threads_per_world": t.threads_per_world, "steps": t.steps,
+                "ms_per_step": dict(zip((64, 128, 256, 512), [round(x, 4) for x in t.ms_per_step]))}
+
+    def set_batched_block(self, threads_per_world: int) -> None:
+        check(self.lib.nans_set_batched_block(self.h, int(threads_per_world)))
 
     def batched_timing(self) -> dict:
         t = Timing()

@@ -144,6 +144,37 @@ def test_plain_equals_batched_graph_replay():
     b.close()
 
 
+def test_workgroup_sizes_and_autotuner():
+    """Every workgroup size of the batched step gives the same bits; the autotuner
+    leaves the state as it found it and its choice steps the same."""
+    sc = _c3_subset()
+    ref = None
+    for tpw in (64, 128, 256, 512):
+        ctx = _ctx(sc)
+        ctx.set_batched_block(tpw)
+        ctx.step_batched(20)
+        assert ctx.batched_timing()["threads_per_world"] == tpw
+        got = ctx.download()
+        ctx.close()
+        if ref is None:
+            ref = got
+        for f in STATE:
+            np.testing.assert_array_equal(got[f], ref[f], err_msg="%d threads %s" % (tpw, f))
+    ctx = _ctx(sc)
+    before = ctx.download()
+    tune = ctx.autotune(3)
+    print("autotune on the config-3 subset:", tune)
+    assert tune["threads_per_world"] in (64, 128, 256, 512)
+    after = ctx.download()
+    for f in STATE:
+        np.testing.assert_array_equal(after[f], before[f])
+    ctx.step_batched(20)
+    got = ctx.download()
+    for f in STATE:
+        np.testing.assert_array_equal(got[f], ref[f])
+    ctx.close()
+
+
 def test_full_config3_properties(oracle_mod):
     """All 4096 config-3 worlds, 240 steps, batched: finite state, no overflow, and
     three sampled worlds (largest, smallest, one median) equal to the oracle."""
