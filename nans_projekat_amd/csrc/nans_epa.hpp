@@ -2,21 +2,28 @@
 // per pair, and the workgroup narrowphase driver (GJK per lane, EPA per wave).
 //
 // Layout of the polytope:
-//   vertices   LDS: P, SupA, SupB and an equality class per vertex (the first
-//              vertex with the same P, 255 if P has a NaN: never equal);
+//   vertices   registers: lane l holds vertex l (and lane l < 8 vertex 64 + l): its
+//              P and its equality class (the first vertex with the same P, 255 if
+//              P has a NaN: never equal); LDS keeps P, SupA, SupB for gathers;
 //   faces      registers: lane l owns faces l, l+64, l+128 and keeps for each its
 //              stored normal N, first vertex A, distance |dot(n, A)| and the packed
 //              ids and classes of its three vertices;
-//   edges      registers: horizon edge e lives in lane e.
+//   edges      registers: horizon edge e lives in lane e;
+//   shapes     registers: lanes 0-7 hold corner (lane) of box A, lanes 8-15 corner
+//              (lane - 8) of box B; sphere centres and radii in every lane.
 // One iteration of the reference's loop (code/nans.cpp:804-902):
 //   closest face   the first face holding the minimum of the distances below face
 //                  0's (the strict '<' scan from face 0, code/nans.cpp:807-822):
 //                  DPP row minimum + ballots, the winner's data by readlane;
-//   new support    along the winner's stored normal (code/nans.cpp:824-825);
+//   new support    along the winner's stored normal (code/nans.cpp:824-825): one
+//                  corner dot product per lane and a DPP argmax over each group of
+//                  8 lanes with the first-index tie rule of the strict '>' scan;
 //   visibility     each lane tests its faces against the new point, ballots;
-//   horizon        the visible faces' edges pushed in face order with the exact
-//                  reversed-edge cancellation of PushEdge (code/nans.cpp:232-266),
-//                  one ballot per push;
+//   horizon        the visible faces' edges in face order with the exact
+//                  reversed-edge cancellation of PushEdge (code/nans.cpp:232-266):
+//                  when no edge key occurs twice, an edge survives iff its reverse
+//                  is not among the pushed edges, which every lane checks at once;
+//                  otherwise (or beyond 64 pushes) the pushes run one at a time;
 //   compaction     survivors keep their order (std::vector::erase), new faces
 //                  follow in edge order: one pass through an LDS staging area.
 // A face's distance and flipped normal never change once it is made, so they are
@@ -36,13 +43,12 @@ struct EpaLds {
     float vp[EPA_MAXV][3];
     float va[EPA_MAXV][3];
     float vb[EPA_MAXV][3];
-    int vcls[EPA_MAXV];
     float sfn[EPA_MAXF][3];  // compaction staging: stored normal
     float sfd[EPA_MAXF];     //                     distance
     int sfid[EPA_MAXF];      //                     a | b << 8 | c << 16
     int sfcl[EPA_MAXF];      //                     classes of a, b, c
-    float sa[32];            // shape A: kind, c.xyz, r, 24 corner floats
-    float sb[32];
+    int ekey[128];           // horizon: pushed edges' classes (0..63), survivors (64..127)
+    int eids[128];           //          and vertex ids, a | b << 8
 };
 
 struct EpaResult {
@@ -62,40 +68,6 @@ __device__ __forceinline__ void wave_sync()
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int ffs64(uint64_t m) { return __builtin_ctzll(m); }
-
-__device__ inline void shape_store(float *dst, const Shape &s)
-{
-    dst[0] = __int_as_float(s.kind);
-    dst[1] = s.c.x; dst[2] = s.c.y; dst[3] = s.c.z;
-    dst[4] = s.r;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        dst[5 + 3 * k] = s.v[k].x;
-        dst[6 + 3 * k] = s.v[k].y;
-        dst[7 + 3 * k] = s.v[k].z;
-    }
-}
-
-// Support of an LDS-resident shape (every lane reads the same words: broadcast).
-__device__ inline V3 support_lds(const float *s, V3 d)
-{
-    int kind = __float_as_int(s[0]);
-    if (kind == SHAPE_BOX) {
-        float maxd = -FLT_MAX;
-        V3 r = mk3(0.0f, 0.0f, 0.0f);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            V3 v = mk3(s[5 + 3 * k], s[6 + 3 * k], s[7 + 3 * k]);
-            float dist = dot(v, d);
-            if (dist > maxd) {
-                maxd = dist;
-                r = v;
-            }
-        }
-        return r;
-    }
-    return sphere_support(mk3(s[1], s[2], s[3]), s[4], d);
-}
 
 __device__ __forceinline__ V3 ld3(const float (*a)[3], int i) { return mk3(a[i][0], a[i][1], a[i][2]); }
 __device__ __forceinline__ void st3(float (*a)[3], int i, V3 v)
@@ -149,20 +121,119 @@ __device__ inline void wave_argmin(bool &valid, T &v, int &i)
     }
 }
 
-// Equality class of vertex id: the first earlier vertex with the same P (vertex
-// == compares P, code/nans.h:251-254), CLS_NAN if P has a NaN (never equal).
-// Every lane must call it.
-__device__ inline int vertex_class(const EpaLds &L, int id, V3 P)
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+
+// The two shapes of a pair as one wave holds them for EPA.
+struct EpaShapes {
+    int kind_a, kind_b;  // uniform
+    V3 ca, cb;           // centres (spheres)
+    float ra, rb;        // radii (spheres)
+    V3 k;                // lanes 0-7: corner (lane) of A; lanes 8-15: corner (lane - 8) of B
+};
+
+__device__ inline V3 pick_corner(const Shape &s, int k)
+{
+    V3 v = s.v[0];
+#pragma unroll
+    for (int j = 1; j < 8; ++j) {
+        const bool t = k == j;
+        v.x = t ? s.v[j].x : v.x;
+        v.y = t ? s.v[j].y : v.y;
+        v.z = t ? s.v[j].z : v.z;
+    }
+    return v;
+}
+
+__device__ inline EpaShapes epa_shapes(const Shape &A, const Shape &B)
+{
+    const int lane = lane_id();
+    EpaShapes e;
+    e.kind_a = A.kind;
+    e.kind_b = B.kind;
+    e.ca = A.c;
+    e.cb = B.c;
+    e.ra = A.r;
+    e.rb = B.r;
+    const V3 ka = pick_corner(A, lane & 7), kb = pick_corner(B, lane & 7);
+    const bool onb = lane >= 8;
+    e.k = mk3(onb ? kb.x : ka.x, onb ? kb.y : ka.y, onb ? kb.z : ka.z);
+    return e;
+}
+
+// Supports of the boxes held as corners in lanes 0-7 (A, along d) and 8-15 (B,
+// along -d), both at once: in each group of 8 lanes, the first corner of greatest
+// dot(v, dir) among those above -FLT_MAX, else (0,0,0) -- GetCubeSupport's strict
+// '>' scan from -FLT_MAX (code/nans.cpp:409-430).
+__device__ __forceinline__ void box_supports(V3 kv, V3 d, V3 nd, V3 &sa, V3 &sb)
+{
+    const int lane = lane_id();
+    const float dist = dot(kv, lane >= 8 ? nd : d);
+    float key = dist > -FLT_MAX ? dist : -__builtin_inff();
+    int idx = lane & 7;
+    auto step = [&](float ok, int oi) {
+        const bool take = ok > key || (ok == key && oi < idx);
+        key = take ? ok : key;
+        idx = take ? oi : idx;
+    };
+    step(dpp_f<0xB1>(key), dpp_i<0xB1>(idx));    // quad_perm [1,0,3,2]
+    step(dpp_f<0x4E>(key), dpp_i<0x4E>(idx));    // quad_perm [2,3,0,1]
+    step(dpp_f<0x141>(key), dpp_i<0x141>(idx));  // row_half_mirror: the other quad of the 8
+    const float ka = readlane_f(key, 0), kb = readlane_f(key, 8);
+    const int wa = readlane_i(idx, 0), wb = 8 + readlane_i(idx, 8);
+    sa = ka == -__builtin_inff() ? mk3(0.0f, 0.0f, 0.0f)
+                                 : mk3(readlane_f(kv.x, wa), readlane_f(kv.y, wa), readlane_f(kv.z, wa));
+    sb = kb == -__builtin_inff() ? mk3(0.0f, 0.0f, 0.0f)
+                                 : mk3(readlane_f(kv.x, wb), readlane_f(kv.y, wb), readlane_f(kv.z, wb));
+}
+
+// CalculateSupport (code/nans.cpp:463-519): S_A(d) and S_B(-d); every lane calls.
+__device__ inline Vtx wave_support(const EpaShapes &s, V3 d)
+{
+    const V3 nd = mul(d, -1.0f);
+    Vtx v;
+    V3 ga = mk3(0.0f, 0.0f, 0.0f), gb = mk3(0.0f, 0.0f, 0.0f);
+    if (s.kind_a == SHAPE_BOX || s.kind_b == SHAPE_BOX) box_supports(s.k, d, nd, ga, gb);
+    v.A = s.kind_a == SHAPE_BOX ? ga : sphere_support(s.ca, s.ra, d);
+    v.B = s.kind_b == SHAPE_BOX ? gb : sphere_support(s.cb, s.rb, nd);
+    v.P = sub(v.A, v.B);
+    return v;
+}
+
+// Position of the n-th set bit of m (n < popcount(m)).
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int n)
+{
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const uint64_t low = m & ((1ull << w) - 1ull);
+        const int c = __popcll(low);
+        if (n >= c) {
+            n -= c;
+            pos += w;
+            m >>= w;
+        } else {
+            m = low;
+        }
+    }
+    return pos;
+}
+
+// Equality class of a new vertex with id nv and point P against the vertices in
+// registers (vertex == compares P, code/nans.h:251-254): the first earlier vertex
+// with the same P, CLS_NAN if P has a NaN.  Every lane calls.
+__device__ __forceinline__ int vertex_class_reg(V3 v0, V3 v1, int nv, V3 P)
 {
     if (has_nan(P)) return CLS_NAN;
-    int lane = lane_id();
-    for (int base = 0; base < id; base += 64) {
-        int u = base + lane;
-        bool m = u < id && veq(ld3(L.vp, u), P);
-        uint64_t mm = ballot(m);
-        if (mm) return base + ffs64(mm);
-    }
-    return id;
+    const int lane = lane_id();
+    const uint64_t m0 = ballot(lane < nv && veq(v0, P));
+    if (m0) return ffs64(m0);
+    const uint64_t m1 = ballot(lane + 64 < nv && veq(v1, P));
+    return m1 ? 64 + ffs64(m1) : nv;
 }
 
 __device__ __forceinline__ float pick3f(int k, float a, float b, float c) { return k == 0 ? a : (k == 1 ? b : c); }
@@ -205,9 +276,10 @@ __device__ inline double face_dist64(const EpaLds &L, const FaceR &f)
 }
 
 // ResolveCollision (code/nans.cpp:787-904) executed by one full wavefront.  The
-// caller has written the shapes to L.sa / L.sb and the GJK simplex to vertices
-// 0..3, then called wave_sync().  Every lane returns the same result.
-__device__ inline EpaResult epa_wave(EpaLds &L, const NarrowParams &p)
+// caller has written the GJK simplex to L.vp / L.va / L.vb[0..3], passes vertex
+// (lane)'s P in p0 for lanes 0-3, and has called wave_sync().  Every lane returns
+// the same result.
+__device__ inline EpaResult epa_wave(EpaLds &L, const EpaShapes &sh, V3 p0, const NarrowParams &p)
 {
     const int lane = lane_id();
     EpaResult r;
@@ -218,24 +290,34 @@ __device__ inline EpaResult epa_wave(EpaLds &L, const NarrowParams &p)
     r.max_faces = 4;
     r.nverts = 4;
 
+    // vertices in registers: v0 = vertex (lane), v1 = vertex (64 + lane)
+    V3 v0 = lane < 4 ? p0 : mk3(0.0f, 0.0f, 0.0f);
+    V3 v1 = mk3(0.0f, 0.0f, 0.0f);
+    int c0 = 0;
     {
-        int cls = 0;
-        for (int v = 0; v < 4; ++v) {
-            int c = vertex_class(L, v, ld3(L.vp, v));
-            if (lane == v) cls = c;
+        // classes of the simplex vertices: the first earlier vertex with the same P
+        int cls = lane;
+        bool found = false;
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const V3 pu = mk3(readlane_f(v0.x, u), readlane_f(v0.y, u), readlane_f(v0.z, u));
+            if (!found && u < lane && veq(pu, v0)) {
+                cls = u;
+                found = true;
+            }
         }
-        if (lane < 4) L.vcls[lane] = cls;
+        c0 = has_nan(v0) ? CLS_NAN : cls;
     }
-    wave_sync();
     FaceR F[EPA_FSLOTS];
 #pragma unroll
     for (int k = 0; k < EPA_FSLOTS; ++k) F[k] = FaceR{0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (lane < 4) {
+    {
         // ABC, ACD, ADB, BDC (code/nans.cpp:799-802)
         const int a = lane == 3 ? 1 : 0;
         const int b = lane == 0 ? 1 : (lane == 1 ? 2 : 3);
         const int c = lane == 0 ? 2 : (lane == 1 ? 3 : (lane == 2 ? 1 : 2));
-        F[0] = make_face(ld3(L.vp, a), ld3(L.vp, b), ld3(L.vp, c), a, b, c, L.vcls[a], L.vcls[b], L.vcls[c]);
+        const int ca = __shfl(c0, a, 64), cb = __shfl(c0, b, 64), cc = __shfl(c0, c, 64);
+        if (lane < 4) F[0] = make_face(ld3(L.vp, a), ld3(L.vp, b), ld3(L.vp, c), a, b, c, ca, cb, cc);
     }
     int nf = 4, nv = 4;
     int iter = 0;
@@ -270,13 +352,11 @@ __device__ inline EpaResult epa_wave(EpaLds &L, const NarrowParams &p)
                    readlane_f(pick3f(ks, F[0].nz, F[1].nz, F[2].nz), owner));
         float cur = m == __builtin_inff() ? d0 : readlane_f(pick3f(ks, F[0].d, F[1].d, F[2].d), owner);
         // new support along the stored normal
-        Vtx ns;
-        ns.A = support_lds(L.sa, N);
-        ns.B = support_lds(L.sb, mul(N, -1.0f));
-        ns.P = sub(ns.A, ns.B);
+        const Vtx ns = wave_support(sh, N);
         if (dot(N, ns.P) - cur < p.epa_tolerance) {
             double depth = (double)cur;
             if (p.fp64_islands) {
+                wave_sync();
                 double d64[EPA_FSLOTS];
 #pragma unroll
                 for (int k = 0; k < EPA_FSLOTS; ++k) d64[k] = (lane + 64 * k < nf) ? face_dist64(L, F[k]) : 0.0;
@@ -308,12 +388,13 @@ __device__ inline EpaResult epa_wave(EpaLds &L, const NarrowParams &p)
             }
             const int id = readlane_i(pick3i(ks, F[0].id, F[1].id, F[2].id), owner);
             const int a = id & 255, b = (id >> 8) & 255, c = (id >> 16) & 255;
+            wave_sync();
             V3 PA = ld3(L.vp, a), PB = ld3(L.vp, b), PC = ld3(L.vp, c);
             // Barycentric (code/nans.cpp:771-785) of N * d
             V3 P = mul(N, cur);
-            V3 v0 = sub(PB, PA), v1 = sub(PC, PA), v2 = sub(P, PA);
-            float d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1);
-            float d20 = dot(v2, v0), d21 = dot(v2, v1);
+            V3 e0 = sub(PB, PA), e1 = sub(PC, PA), e2 = sub(P, PA);
+            float d00 = dot(e0, e0), d01 = dot(e0, e1), d11 = dot(e1, e1);
+            float d20 = dot(e2, e0), d21 = dot(e2, e1);
             float denom = d00 * d11 - d01 * d01;
             float V = (d11 * d20 - d01 * d21) / denom;
             float W = (d00 * d21 - d01 * d20) / denom;
@@ -337,57 +418,130 @@ __device__ inline EpaResult epa_wave(EpaLds &L, const NarrowParams &p)
         }
         // append the vertex
         const int vnew = nv;
-        const int ncls = vertex_class(L, vnew, ns.P);
+        const int ncls = vertex_class_reg(v0, v1, vnew, ns.P);
+        if (lane == (vnew & 63)) {
+            if (vnew < 64) v0 = ns.P;
+            else v1 = ns.P;
+        }
         if (lane == 0) {
             st3(L.vp, vnew, ns.P);
             st3(L.va, vnew, ns.A);
             st3(L.vb, vnew, ns.B);
-            L.vcls[vnew] = ncls;
         }
         nv += 1;
         r.nverts = nv;
         // visibility (code/nans.cpp:869-891): the recomputed flipped normal is the stored one
         uint64_t vis[EPA_FSLOTS];
+        int nvis = 0;
 #pragma unroll
         for (int k = 0; k < EPA_FSLOTS; ++k) {
             const int f = lane + 64 * k;
             const bool vv = f < nf && dot(mk3(F[k].nx, F[k].ny, F[k].nz),
                                           sub(ns.P, mk3(F[k].ax, F[k].ay, F[k].az))) > 0.0f;
             vis[k] = ballot(vv);
+            nvis += __popcll(vis[k]);
         }
-        // horizon edges in face order (PushEdge), one edge per lane
+        // horizon edges in face order (PushEdge), edge e in lane e
         int ne = 0, ea = 0, eb = 0, eca = 0, ecb = 0;
-        bool overflow = false;
+        // fast path (all faces in one slot, at most 64 pushes): an edge survives iff
+        // its reverse is not among the pushed edges, provided no key occurs twice
+        bool sequential = nf > 64 || 3 * nvis > 64;
+        if (!sequential) {
+            const bool mine = (vis[0] >> lane) & 1ull;
+            const int cl = F[0].cl;
+            int key[3], rev[3], nsame[3] = {0, 0, 0}, nrev[3] = {0, 0, 0};
 #pragma unroll
-        for (int k = 0; k < EPA_FSLOTS; ++k) {
-            uint64_t mk = vis[k];
+            for (int e = 0; e < 3; ++e) {
+                const int ca = (cl >> (8 * e)) & 255, cb = (cl >> (8 * ((e + 1) % 3))) & 255;
+                key[e] = ca | (cb << 8);
+                rev[e] = (ca == CLS_NAN || cb == CLS_NAN) ? -1 : (cb | (ca << 8));
+            }
+            uint64_t mk = vis[0];
             while (mk) {
                 const int o = ffs64(mk);
                 mk &= mk - 1;
-                const int id = readlane_i(F[k].id, o), cl = readlane_i(F[k].cl, o);
-                const int fv[3] = {id & 255, (id >> 8) & 255, (id >> 16) & 255};
-                const int fc[3] = {cl & 255, (cl >> 8) & 255, (cl >> 16) & 255};
+                const int co = readlane_i(cl, o);
 #pragma unroll
-                for (int e = 0; e < 3; ++e) {
-                    const int A = fv[e], B = fv[(e + 1) % 3];
-                    const int cA = fc[e], cB = fc[(e + 1) % 3];
-                    const bool match = lane < ne && eca != CLS_NAN && eca == cB && ecb != CLS_NAN && ecb == cA;
-                    const uint64_t mm = ballot(match);
-                    if (mm) {
-                        const int j = ffs64(mm);
-                        const int na = __shfl_down(ea, 1, 64), nb = __shfl_down(eb, 1, 64);
-                        const int nca = __shfl_down(eca, 1, 64), ncb = __shfl_down(ecb, 1, 64);
-                        if (lane >= j) {
-                            ea = na; eb = nb; eca = nca; ecb = ncb;
+                for (int t = 0; t < 3; ++t) {
+                    const int kt = ((co >> (8 * t)) & 255) | (((co >> (8 * ((t + 1) % 3))) & 255) << 8);
+#pragma unroll
+                    for (int e = 0; e < 3; ++e) {
+                        nsame[e] += kt == key[e];
+                        nrev[e] += kt == rev[e];
+                    }
+                }
+            }
+            bool dup = false;
+#pragma unroll
+            for (int e = 0; e < 3; ++e) dup |= rev[e] >= 0 && (nsame[e] > 1 || nrev[e] > 1);
+            sequential = ballot(mine && dup) != 0ull;
+            if (!sequential) {
+                // kept edges of this lane's face, placed after those of earlier visible faces
+                int keep = 0;
+#pragma unroll
+                for (int e = 0; e < 3; ++e) keep |= (rev[e] < 0 || nrev[e] == 0) ? (1 << e) : 0;
+                const int cnt = mine ? __popc(keep) : 0;
+                const uint64_t lower = (1ull << lane) - 1ull;
+                const uint64_t b0 = ballot(cnt & 1), b1 = ballot(cnt & 2);
+                int o = __popcll(b0 & lower) + 2 * __popcll(b1 & lower);
+                ne = __popcll(b0) + 2 * __popcll(b1);
+                if (mine) {
+                    const int id = F[0].id;
+#pragma unroll
+                    for (int e = 0; e < 3; ++e) {
+                        if ((keep >> e) & 1) {
+                            const int s0 = 8 * e, s1 = 8 * ((e + 1) % 3);
+                            L.eids[o] = ((id >> s0) & 255) | (((id >> s1) & 255) << 8);
+                            L.ekey[o] = key[e];
+                            ++o;
                         }
-                        ne -= 1;
-                    } else if (ne < 64) {
-                        if (lane == ne) {
-                            ea = A; eb = B; eca = cA; ecb = cB;
+                    }
+                }
+                wave_sync();
+                if (lane < ne) {
+                    const int k2 = L.ekey[lane], i2 = L.eids[lane];
+                    ea = i2 & 255;
+                    eb = (i2 >> 8) & 255;
+                    eca = k2 & 255;
+                    ecb = (k2 >> 8) & 255;
+                }
+                wave_sync();
+            }
+        }
+        bool overflow = false;
+        if (sequential) {
+            ne = 0;
+#pragma unroll
+            for (int k = 0; k < EPA_FSLOTS; ++k) {
+                uint64_t mk = vis[k];
+                while (mk) {
+                    const int o = ffs64(mk);
+                    mk &= mk - 1;
+                    const int id = readlane_i(F[k].id, o), cl = readlane_i(F[k].cl, o);
+                    const int fv[3] = {id & 255, (id >> 8) & 255, (id >> 16) & 255};
+                    const int fc[3] = {cl & 255, (cl >> 8) & 255, (cl >> 16) & 255};
+#pragma unroll
+                    for (int e = 0; e < 3; ++e) {
+                        const int A = fv[e], B = fv[(e + 1) % 3];
+                        const int cA = fc[e], cB = fc[(e + 1) % 3];
+                        const bool match = lane < ne && eca != CLS_NAN && eca == cB && ecb != CLS_NAN && ecb == cA;
+                        const uint64_t mm = ballot(match);
+                        if (mm) {
+                            const int j = ffs64(mm);
+                            const int na = __shfl_down(ea, 1, 64), nb = __shfl_down(eb, 1, 64);
+                            const int nca = __shfl_down(eca, 1, 64), ncb = __shfl_down(ecb, 1, 64);
+                            if (lane >= j) {
+                                ea = na; eb = nb; eca = nca; ecb = ncb;
+                            }
+                            ne -= 1;
+                        } else if (ne < 64) {
+                            if (lane == ne) {
+                                ea = A; eb = B; eca = cA; ecb = cB;
+                            }
+                            ne += 1;
+                        } else {
+                            overflow = true;
                         }
-                        ne += 1;
-                    } else {
-                        overflow = true;
                     }
                 }
             }
@@ -396,14 +550,16 @@ __device__ inline EpaResult epa_wave(EpaLds &L, const NarrowParams &p)
             r.status = EPA_OVERFLOW;
             break;
         }
-        // ordered removal of the visible faces, then the new ones, through LDS
+        // ordered removal of the visible faces, then the new ones
         int nsurv = 0;
         int dst[EPA_FSLOTS];
+        uint64_t am0 = 0ull;
 #pragma unroll
         for (int k = 0; k < EPA_FSLOTS; ++k) {
             const int f = lane + 64 * k;
             const bool alive = f < nf && !((vis[k] >> lane) & 1ull);
             const uint64_t am = ballot(alive);
+            if (k == 0) am0 = am;
             dst[k] = alive ? nsurv + __popcll(am & ((1ull << lane) - 1ull)) : -1;
             nsurv += __popcll(am);
         }
@@ -411,6 +567,42 @@ __device__ inline EpaResult epa_wave(EpaLds &L, const NarrowParams &p)
             r.status = EPA_OVERFLOW;
             break;
         }
+        // PushTriangle(NewSupport, e.A, e.B) (code/nans.cpp:894-899) in lane e
+        FaceR g = FaceR{0, 0, 0, 0, 0, 0, 0, 0, 0};
+        wave_sync();  // lane 0's vertex stores before other lanes read L.vp
+        if (lane < ne) g = make_face(ns.P, ld3(L.vp, ea), ld3(L.vp, eb), vnew, ea, eb, ncls, eca, ecb);
+        const int nfn = nsurv + ne;
+        if (nf <= 64 && nfn <= 64) {
+            // lane f takes survivor f from its lane, or new face f - nsurv from its edge lane
+            const bool old = lane < nsurv;
+            const int src = old ? nth_set_bit(am0, lane) : lane - nsurv;
+            const int adr = 4 * (src & 63);
+            auto pull = [&](float a_, float b_) {
+                const float x = __int_as_float(__builtin_amdgcn_ds_bpermute(adr, __float_as_int(a_)));
+                const float y = __int_as_float(__builtin_amdgcn_ds_bpermute(adr, __float_as_int(b_)));
+                return old ? x : y;
+            };
+            auto pulli = [&](int a_, int b_) {
+                const int x = __builtin_amdgcn_ds_bpermute(adr, a_);
+                const int y = __builtin_amdgcn_ds_bpermute(adr, b_);
+                return old ? x : y;
+            };
+            FaceR h;
+            h.nx = pull(F[0].nx, g.nx);
+            h.ny = pull(F[0].ny, g.ny);
+            h.nz = pull(F[0].nz, g.nz);
+            h.ax = pull(F[0].ax, g.ax);
+            h.ay = pull(F[0].ay, g.ay);
+            h.az = pull(F[0].az, g.az);
+            h.d = pull(F[0].d, g.d);
+            h.id = pulli(F[0].id, g.id);
+            h.cl = pulli(F[0].cl, g.cl);
+            F[0] = h;
+            nf = nfn;
+            if (nf > r.max_faces) r.max_faces = nf;
+            continue;
+        }
+        // otherwise through the LDS staging area
 #pragma unroll
         for (int k = 0; k < EPA_FSLOTS; ++k) {
             if (dst[k] >= 0) {
@@ -423,8 +615,6 @@ __device__ inline EpaResult epa_wave(EpaLds &L, const NarrowParams &p)
             }
         }
         if (lane < ne) {
-            // PushTriangle(NewSupport, e.A, e.B) (code/nans.cpp:894-899)
-            FaceR g = make_face(ns.P, ld3(L.vp, ea), ld3(L.vp, eb), vnew, ea, eb, ncls, eca, ecb);
             const int s = nsurv + lane;
             L.sfn[s][0] = g.nx;
             L.sfn[s][1] = g.ny;
@@ -433,7 +623,7 @@ __device__ inline EpaResult epa_wave(EpaLds &L, const NarrowParams &p)
             L.sfid[s] = g.id;
             L.sfcl[s] = g.cl;
         }
-        nf = nsurv + ne;
+        nf = nfn;
         if (nf > r.max_faces) r.max_faces = nf;
         wave_sync();
 #pragma unroll
@@ -451,6 +641,356 @@ __device__ inline EpaResult epa_wave(EpaLds &L, const NarrowParams &p)
                 F[k].ay = L.vp[a][1];
                 F[k].az = L.vp[a][2];
             }
+        }
+        wave_sync();
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------- four EPAs per wave
+// ResolveCollision for four pairs at once, one per 16-lane group (a row of the
+// wave): the same steps as epa_wave, with the polytope of each pair held by its
+// group -- faces f and 16 + f in lane f, vertices v and 16 + v in lane v (P, SupA,
+// SupB), P also in LDS for gathers -- group reductions by row-local DPP and group
+// broadcasts by ds_bpermute.  A polytope that outgrows a group (33 vertices, 33
+// faces, 17 horizon edges, or an edge key pushed twice) stops with EPA_SPILL and is
+// redone from its simplex by epa_wave, so the result is always epa_wave's.
+constexpr int EPA_SPILL = 99;
+constexpr int EPG_MAXV = 32;
+constexpr int EPG_MAXF = 32;
+
+struct EpgLds {
+    float vp[4][EPG_MAXV][3];      // vertex P per group
+    float sf[4][EPG_MAXF][9];      // compaction staging: n, a, d, id, cl
+    int ekey[64];                  // horizon edges per group (16 each)
+    int eids[64];
+};
+static_assert(sizeof(EpgLds) <= sizeof(EpaLds), "the grouped EPA reuses the wave's EPA LDS");
+
+__device__ __forceinline__ float gpull_f(float v, int src_lane)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src_lane << 2, __float_as_int(v)));
+}
+__device__ __forceinline__ int gpull_i(int v, int src_lane) { return __builtin_amdgcn_ds_bpermute(src_lane << 2, v); }
+__device__ __forceinline__ V3 gpull3(V3 v, int src_lane)
+{
+    return mk3(gpull_f(v.x, src_lane), gpull_f(v.y, src_lane), gpull_f(v.z, src_lane));
+}
+// this lane's group bits of a wave ballot
+__device__ __forceinline__ uint32_t gbits(uint64_t m) { return (uint32_t)(m >> (lane_id() & 48)) & 0xffffu; }
+// minimum over the lane's row of 16 (no NaN inputs), in every lane of the row
+__device__ __forceinline__ float row_min(float v)
+{
+    v = fminf(v, dpp_f<0xB1>(v));
+    v = fminf(v, dpp_f<0x4E>(v));
+    v = fminf(v, dpp_f<0x141>(v));
+    v = fminf(v, dpp_f<0x140>(v));
+    return v;
+}
+__device__ __forceinline__ int nth_set_bit32(uint32_t m, int n) { return nth_set_bit((uint64_t)m, n); }
+
+// The group's shapes: lanes 0-7 of each group hold the corners of A, 8-15 of B.
+__device__ inline EpaShapes epa_shapes_group(const Shape &A, const Shape &B)
+{
+    const int j = lane_id() & 15;
+    EpaShapes e;
+    e.kind_a = A.kind;
+    e.kind_b = B.kind;
+    e.ca = A.c;
+    e.cb = B.c;
+    e.ra = A.r;
+    e.rb = B.r;
+    const V3 ka = pick_corner(A, j & 7), kb = pick_corner(B, j & 7);
+    const bool onb = j >= 8;
+    e.k = mk3(onb ? kb.x : ka.x, onb ? kb.y : ka.y, onb ? kb.z : ka.z);
+    return e;
+}
+
+// CalculateSupport for each group's pair along its d (group-uniform).
+__device__ inline Vtx group_support(const EpaShapes &s, V3 d)
+{
+    const int lane = lane_id(), j = lane & 15, gb = lane & 48;
+    const V3 nd = mul(d, -1.0f);
+    const float dist = dot(s.k, j >= 8 ? nd : d);
+    float key = dist > -FLT_MAX ? dist : -__builtin_inff();
+    int idx = j & 7;
+    auto step = [&](float ok, int oi) {
+        const bool take = ok > key || (ok == key && oi < idx);
+        key = take ? ok : key;
+        idx = take ? oi : idx;
+    };
+    step(dpp_f<0xB1>(key), dpp_i<0xB1>(idx));
+    step(dpp_f<0x4E>(key), dpp_i<0x4E>(idx));
+    step(dpp_f<0x141>(key), dpp_i<0x141>(idx));
+    const float ka = gpull_f(key, gb), kb = gpull_f(key, gb + 8);
+    const int wa = gb + gpull_i(idx, gb), wb = gb + 8 + gpull_i(idx, gb + 8);
+    const V3 ca = gpull3(s.k, wa), cb = gpull3(s.k, wb);
+    Vtx v;
+    v.A = s.kind_a == SHAPE_BOX ? (ka == -__builtin_inff() ? mk3(0.0f, 0.0f, 0.0f) : ca)
+                                : sphere_support(s.ca, s.ra, d);
+    v.B = s.kind_b == SHAPE_BOX ? (kb == -__builtin_inff() ? mk3(0.0f, 0.0f, 0.0f) : cb)
+                                : sphere_support(s.cb, s.rb, nd);
+    v.P = sub(v.A, v.B);
+    return v;
+}
+
+// Every lane calls; lanes 0-3 of a group pass the supports of simplex vertex j in
+// sa0 / sb0.  act: the group has a pair.  Returns the group's result (EPA_NONE for
+// an idle group, EPA_SPILL to redo with epa_wave).
+__device__ inline EpaResult epa_group(EpgLds &L, const EpaShapes &sh, V3 sa0, V3 sb0, bool act,
+                                      const NarrowParams &p)
+{
+    const int lane = lane_id(), j = lane & 15, gb = lane & 48, g = lane >> 4;
+    EpaResult r;
+    r.status = act ? EPA_ITER : EPA_NONE;
+    r.pa = r.pb = r.n = mk3(0.0f, 0.0f, 0.0f);
+    r.depth = 0.0;
+    r.iters = 0;
+    r.max_faces = 4;
+    r.nverts = 4;
+    bool run = act;
+    const V3 z = mk3(0.0f, 0.0f, 0.0f);
+    // vertices v (slot 0) and 16 + v (slot 1) of lane v
+    V3 VA0 = j < 4 ? sa0 : z, VB0 = j < 4 ? sb0 : z;
+    V3 VP0 = j < 4 ? sub(sa0, sb0) : z;
+    V3 VA1 = z, VB1 = z, VP1 = z;
+    if (j < 4) st3(L.vp[g], j, VP0);
+    int c0;
+    {
+        int cls = j;
+        bool found = false;
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const V3 pu = gpull3(VP0, gb + u);
+            if (!found && u < j && veq(pu, VP0)) {
+                cls = u;
+                found = true;
+            }
+        }
+        c0 = has_nan(VP0) ? CLS_NAN : cls;
+    }
+    FaceR F0 = FaceR{0, 0, 0, 0, 0, 0, 0, 0, 0}, F1 = F0;
+    {
+        const int a = j == 3 ? 1 : 0;
+        const int b = j == 0 ? 1 : (j == 1 ? 2 : 3);
+        const int c = j == 0 ? 2 : (j == 1 ? 3 : (j == 2 ? 1 : 2));
+        const V3 Pa = gpull3(VP0, gb + a), Pb = gpull3(VP0, gb + b), Pc = gpull3(VP0, gb + c);
+        const int ca = gpull_i(c0, gb + a), cb = gpull_i(c0, gb + b), cc = gpull_i(c0, gb + c);
+        if (j < 4) F0 = make_face(Pa, Pb, Pc, a, b, c, ca, cb, cc);
+    }
+    int nf = 4, nv = 4, iter = 0;
+    for (;;) {
+        // while (iter++ <= MAX_EPA_ITERATIONS) per group
+        if (run && !(iter <= p.max_epa_iterations)) run = false;  // status stays EPA_ITER
+        iter += 1;
+        if (run) r.iters += 1;
+        if (run && nf == 0) {
+            r.status = EPA_EMPTY;
+            run = false;
+        }
+        if (!ballot(run)) break;
+        // closest face
+        const float d0 = gpull_f(gabs(dot(mk3(F0.nx, F0.ny, F0.nz), mk3(F0.ax, F0.ay, F0.az))), gb);
+        const float k0 = (j < nf && F0.d < d0) ? F0.d : __builtin_inff();
+        const float k1 = (16 + j < nf && F1.d < d0) ? F1.d : __builtin_inff();
+        const float m = row_min(fminf(k0, k1));
+        const uint32_t b0 = gbits(ballot(k0 == m)), b1 = gbits(ballot(k1 == m));
+        int best = 0;
+        if (m != __builtin_inff()) best = b0 ? ffs64(b0) : (b1 ? 16 + ffs64(b1) : 0);
+        const int owner = gb + (best & 15);
+        const bool ks = best >= 16;
+        const V3 N = mk3(gpull_f(ks ? F1.nx : F0.nx, owner), gpull_f(ks ? F1.ny : F0.ny, owner),
+                         gpull_f(ks ? F1.nz : F0.nz, owner));
+        const float dsel = gpull_f(ks ? F1.d : F0.d, owner);
+        const float cur = m == __builtin_inff() ? d0 : dsel;
+        const int fid = gpull_i(ks ? F1.id : F0.id, owner);
+        // new support along the stored normal
+        const Vtx ns = group_support(sh, N);
+        const bool conv = run && dot(N, ns.P) - cur < p.epa_tolerance;
+        if (ballot(conv)) {
+            // Barycentric (code/nans.cpp:771-785) of N * d on the chosen face
+            const int a = fid & 255, b = (fid >> 8) & 255, c = (fid >> 16) & 255;
+            auto vsel = [&](V3 s0, V3 s1, int v) {
+                const bool hi = v >= 16;
+                return gpull3(mk3(hi ? s1.x : s0.x, hi ? s1.y : s0.y, hi ? s1.z : s0.z), gb + (v & 15));
+            };
+            const V3 PA = vsel(VP0, VP1, a), PB = vsel(VP0, VP1, b), PC = vsel(VP0, VP1, c);
+            const V3 AA = vsel(VA0, VA1, a), AB = vsel(VA0, VA1, b), AC = vsel(VA0, VA1, c);
+            const V3 BA = vsel(VB0, VB1, a), BB = vsel(VB0, VB1, b), BC = vsel(VB0, VB1, c);
+            if (conv) {
+                const V3 P = mul(N, cur);
+                const V3 e0 = sub(PB, PA), e1 = sub(PC, PA), e2 = sub(P, PA);
+                const float d00 = dot(e0, e0), d01 = dot(e0, e1), d11 = dot(e1, e1);
+                const float d20 = dot(e2, e0), d21 = dot(e2, e1);
+                const float denom = d00 * d11 - d01 * d01;
+                const float V = (d11 * d20 - d01 * d21) / denom;
+                const float W = (d00 * d21 - d01 * d20) / denom;
+                const float U = 1.0f - V - W;
+                if (fabsf(U) > 1.0f || fabsf(V) > 1.0f || fabsf(W) > 1.0f) {
+                    r.status = EPA_BARY;
+                } else if (!isfinite(U) || !isfinite(V) || !isfinite(W)) {
+                    r.status = EPA_NONFINITE;
+                } else {
+                    r.pa = add(add(mul(AA, U), mul(AB, V)), mul(AC, W));
+                    r.n = mul(N, -1.0f);
+                    r.pb = add(add(mul(BA, U), mul(BB, V)), mul(BC, W));
+                    r.depth = (double)cur;
+                    r.status = EPA_CONTACT;
+                }
+                run = false;
+            }
+        }
+        if (run && nv >= EPG_MAXV) {
+            r.status = EPA_SPILL;
+            run = false;
+        }
+        // append the vertex (its class: the first earlier vertex with the same P)
+        const int vnew = nv;
+        int ncls;
+        {
+            const uint32_t m0 = gbits(ballot(j < nv && veq(VP0, ns.P)));
+            const uint32_t m1 = gbits(ballot(16 + j < nv && veq(VP1, ns.P)));
+            ncls = has_nan(ns.P) ? CLS_NAN : (m0 ? ffs64(m0) : (m1 ? 16 + ffs64(m1) : nv));
+        }
+        if (run) {
+            if (j == (vnew & 15)) {
+                if (vnew < 16) {
+                    VP0 = ns.P; VA0 = ns.A; VB0 = ns.B;
+                } else {
+                    VP1 = ns.P; VA1 = ns.A; VB1 = ns.B;
+                }
+            }
+            if (j == 0) st3(L.vp[g], vnew, ns.P);
+            nv += 1;
+            r.nverts = nv;
+        }
+        // visibility
+        const bool w0 = run && j < nf && dot(mk3(F0.nx, F0.ny, F0.nz), sub(ns.P, mk3(F0.ax, F0.ay, F0.az))) > 0.0f;
+        const bool w1 = run && 16 + j < nf && dot(mk3(F1.nx, F1.ny, F1.nz), sub(ns.P, mk3(F1.ax, F1.ay, F1.az))) > 0.0f;
+        const uint32_t vis0 = gbits(ballot(w0)), vis1 = gbits(ballot(w1));
+        const uint32_t vis = vis0 | (vis1 << 16);
+        const int nvis = __popc(vis);
+        // horizon: edges of the visible faces in face order; an edge survives iff its
+        // reverse is not among them (no key may occur twice, else spill)
+        int key[2][3], rev[2][3], nsame[2][3], nrev[2][3];
+#pragma unroll
+        for (int sl = 0; sl < 2; ++sl) {
+            const int cl = sl ? F1.cl : F0.cl;
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                const int ca = (cl >> (8 * e)) & 255, cb = (cl >> (8 * ((e + 1) % 3))) & 255;
+                key[sl][e] = ca | (cb << 8);
+                rev[sl][e] = (ca == CLS_NAN || cb == CLS_NAN) ? -1 : (cb | (ca << 8));
+                nsame[sl][e] = 0;
+                nrev[sl][e] = 0;
+            }
+        }
+        const int tmax = max(max(readlane_i(nvis, 0), readlane_i(nvis, 16)), max(readlane_i(nvis, 32), readlane_i(nvis, 48)));
+        for (int t = 0; t < tmax; ++t) {
+            const bool has = t < nvis;
+            const int f = has ? nth_set_bit32(vis, t) : 0;
+            const int co = gpull_i(f >= 16 ? F1.cl : F0.cl, gb + (f & 15));
+            if (has) {
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    const int kt = ((co >> (8 * u)) & 255) | (((co >> (8 * ((u + 1) % 3))) & 255) << 8);
+#pragma unroll
+                    for (int sl = 0; sl < 2; ++sl) {
+#pragma unroll
+                        for (int e = 0; e < 3; ++e) {
+                            nsame[sl][e] += kt == key[sl][e];
+                            nrev[sl][e] += kt == rev[sl][e];
+                        }
+                    }
+                }
+            }
+        }
+        bool dup = false;
+        int keep[2] = {0, 0};
+#pragma unroll
+        for (int sl = 0; sl < 2; ++sl) {
+            const bool mine = sl ? w1 : w0;
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                dup |= mine && rev[sl][e] >= 0 && (nsame[sl][e] > 1 || nrev[sl][e] > 1);
+                keep[sl] |= (mine && (rev[sl][e] < 0 || nrev[sl][e] == 0)) ? (1 << e) : 0;
+            }
+        }
+        const int cnt0 = __popc(keep[0]), cnt1 = __popc(keep[1]);
+        const uint32_t lower = (1u << j) - 1u;
+        const uint32_t p00 = gbits(ballot(cnt0 & 1)), p01 = gbits(ballot(cnt0 & 2));
+        const uint32_t p10 = gbits(ballot(cnt1 & 1)), p11 = gbits(ballot(cnt1 & 2));
+        const int tot0 = __popc(p00) + 2 * __popc(p01);
+        const int ne = tot0 + __popc(p10) + 2 * __popc(p11);
+        if (run && (gbits(ballot(dup)) != 0u || ne > 16)) {
+            r.status = EPA_SPILL;
+            run = false;
+        }
+        if (run) {
+            int o0 = __popc(p00 & lower) + 2 * __popc(p01 & lower);
+            int o1 = tot0 + __popc(p10 & lower) + 2 * __popc(p11 & lower);
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                const int id = sl ? F1.id : F0.id;
+                int &o = sl ? o1 : o0;
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    if ((keep[sl] >> e) & 1) {
+                        const int s0 = 8 * e, s1 = 8 * ((e + 1) % 3);
+                        L.eids[gb + o] = ((id >> s0) & 255) | (((id >> s1) & 255) << 8);
+                        L.ekey[gb + o] = key[sl][e];
+                        ++o;
+                    }
+                }
+            }
+        }
+        wave_sync();
+        int ea = 0, eb = 0, eca = 0, ecb = 0;
+        if (run && j < ne) {
+            const int k2 = L.ekey[gb + j], i2 = L.eids[gb + j];
+            ea = i2 & 255;
+            eb = (i2 >> 8) & 255;
+            eca = k2 & 255;
+            ecb = (k2 >> 8) & 255;
+        }
+        // ordered removal of the visible faces, then the new ones, through LDS
+        const bool al0 = j < nf && !w0, al1 = 16 + j < nf && !w1;
+        const uint32_t alive = gbits(ballot(al0)) | (gbits(ballot(al1)) << 16);
+        const int nsurv = __popc(alive);
+        if (run && nsurv + ne > EPG_MAXF) {
+            r.status = EPA_SPILL;
+            run = false;
+        }
+        FaceR gn = FaceR{0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (run && j < ne) gn = make_face(ns.P, ld3(L.vp[g], ea), ld3(L.vp[g], eb), vnew, ea, eb, ncls, eca, ecb);
+        auto stage = [&](int at, const FaceR &f) {
+            float *d = L.sf[g][at];
+            d[0] = f.nx; d[1] = f.ny; d[2] = f.nz;
+            d[3] = f.ax; d[4] = f.ay; d[5] = f.az;
+            d[6] = f.d;
+            d[7] = __int_as_float(f.id);
+            d[8] = __int_as_float(f.cl);
+        };
+        if (run) {
+            if (al0) stage(__popc(alive & lower), F0);
+            if (al1) stage(__popc(alive & ((lower << 16) | 0xffffu)), F1);
+            if (j < ne) stage(nsurv + j, gn);
+        }
+        wave_sync();
+        if (run) {
+            const int nfn = nsurv + ne;
+            auto unstage = [&](int at, FaceR &f) {
+                const float *d = L.sf[g][at];
+                f.nx = d[0]; f.ny = d[1]; f.nz = d[2];
+                f.ax = d[3]; f.ay = d[4]; f.az = d[5];
+                f.d = d[6];
+                f.id = __float_as_int(d[7]);
+                f.cl = __float_as_int(d[8]);
+            };
+            if (j < nfn) unstage(j, F0);
+            if (16 + j < nfn) unstage(16 + j, F1);
+            nf = nfn;
+            if (nf > r.max_faces) r.max_faces = nf;
         }
         wave_sync();
     }
@@ -549,6 +1089,60 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
     __syncthreads();
     const int nepa = sh.nepa;
     EpaLds &L = epa[threadIdx.x >> 6];
+    if (!np.fp64_islands) {
+        // four pairs per wave, one per 16-lane group; a spilled polytope is redone
+        // by the whole wave
+        EpgLds &G = *reinterpret_cast<EpgLds *>(&L);
+        const int j = lane & 15;
+        for (;;) {
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&sh.epa_next, 4);
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (base >= nepa) break;
+            const int job = base + (lane >> 4);
+            const bool act = job < nepa;
+            const int jq = act ? job : base;
+            const int c = qidx[jq];
+            Shape A, B;
+            src.shapes(c, A, B);
+            const EpaShapes es = epa_shapes_group(A, B);
+            V3 sa0 = mk3(0.0f, 0.0f, 0.0f), sb0 = sa0;
+            if (j < 4) {
+                const SimplexRec &q = qsim[jq];
+                sa0 = mk3(q.a[3 * j], q.a[3 * j + 1], q.a[3 * j + 2]);
+                sb0 = mk3(q.b[3 * j], q.b[3 * j + 1], q.b[3 * j + 2]);
+            }
+            wave_sync();
+            const EpaResult r = epa_group(G, es, sa0, sb0, act, np);
+            if (act && j == 0 && r.status != EPA_SPILL) src.epa_result(c, r);
+            uint64_t spill = ballot(act && j == 0 && r.status == EPA_SPILL);
+            wave_sync();
+            while (spill) {
+                const int sj = base + (ffs64(spill) >> 4);
+                spill &= spill - 1;
+                const int cs = qidx[sj];
+                Shape SA, SB;
+                src.shapes(cs, SA, SB);
+                const EpaShapes ew = epa_shapes(SA, SB);
+                V3 p0 = mk3(0.0f, 0.0f, 0.0f);
+                if (lane < 4) {
+                    const SimplexRec &q = qsim[sj];
+                    V3 a = mk3(q.a[3 * lane], q.a[3 * lane + 1], q.a[3 * lane + 2]);
+                    V3 b = mk3(q.b[3 * lane], q.b[3 * lane + 1], q.b[3 * lane + 2]);
+                    p0 = sub(a, b);
+                    st3(L.va, lane, a);
+                    st3(L.vb, lane, b);
+                    st3(L.vp, lane, p0);
+                }
+                wave_sync();
+                const EpaResult rw = epa_wave(L, ew, p0, np);
+                if (lane == 0) src.epa_result(cs, rw);
+                wave_sync();
+            }
+        }
+        __syncthreads();
+        return;
+    }
     for (;;) {
         int job = 0;
         if (lane == 0) job = atomicAdd(&sh.epa_next, 1);
@@ -557,20 +1151,19 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
         const int c = qidx[job];
         Shape A, B;
         src.shapes(c, A, B);
-        if (lane == 0) {
-            shape_store(L.sa, A);
-            shape_store(L.sb, B);
-        }
+        const EpaShapes es = epa_shapes(A, B);
+        V3 p0 = mk3(0.0f, 0.0f, 0.0f);
         if (lane < 4) {
             const SimplexRec &q = qsim[job];
             V3 a = mk3(q.a[3 * lane], q.a[3 * lane + 1], q.a[3 * lane + 2]);
             V3 b = mk3(q.b[3 * lane], q.b[3 * lane + 1], q.b[3 * lane + 2]);
+            p0 = sub(a, b);
             st3(L.va, lane, a);
             st3(L.vb, lane, b);
-            st3(L.vp, lane, sub(a, b));
+            st3(L.vp, lane, p0);
         }
         wave_sync();
-        const EpaResult r = epa_wave(L, np);
+        const EpaResult r = epa_wave(L, es, p0, np);
         if (lane == 0) src.epa_result(c, r);
         wave_sync();
     }

@@ -360,7 +360,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
 // ------------------------------------------------------------------ batched kernel
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_batched_step(BatchArgs a)
+__global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgs a)
 {
     extern __shared__ float smem[];
     constexpr int NW = BLOCK / 64;
