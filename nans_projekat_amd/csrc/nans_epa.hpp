@@ -666,6 +666,7 @@ struct EpgLds {
     int eids[64];
 };
 static_assert(sizeof(EpgLds) <= sizeof(EpaLds), "the grouped EPA reuses the wave's EPA LDS");
+static_assert(13 * 4 * 64 <= sizeof(EpaLds), "the GJK cycle check keeps 13 words per lane in the EPA LDS");
 
 __device__ __forceinline__ float gpull_f(float v, int src_lane)
 {
@@ -1042,10 +1043,19 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
     }
     __syncthreads();
     {
+        // GJK cycles: EvolveSimplex's next call depends only on the simplex's
+        // Minkowski points (SupA/SupB ride along), so once the points repeat bit for
+        // bit the calls repeat for ever and the loop can only end at the cap.  Each
+        // lane keeps one earlier state (Brent's scheme: saved after calls 1, 2, 4,
+        // 8, ...) in the EPA LDS, which is idle until the GJK phase ends.
+        float *cyc = reinterpret_cast<float *>(epa);  // [13][BLOCK]: P of s0..s3, n
+        const int tid = threadIdx.x;
         Shape A, B;
         Simplex S;
         S.n = 0;
         int c = -1, calls = 0;
+        int power = 1, lam = 0;
+        bool saved = false;
         bool retired = false;
         for (;;) {
             const bool need = c < 0 && !retired;
@@ -1061,6 +1071,7 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
                         src.shapes(c, A, B);
                         S.n = 0;
                         calls = 0;
+                        saved = false;
                     } else {
                         retired = true;
                     }
@@ -1068,8 +1079,28 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
             }
             if (!ballot(c >= 0)) break;
             if (c >= 0) {
-                const int res = evolve_simplex(A, B, S, np.fp64_islands);
+                int res = evolve_simplex(A, B, S, np.fp64_islands);
                 ++calls;
+                if (res == GJK_STILL && calls <= np.max_gjk_iterations) {
+                    const float st[13] = {S.s0.P.x, S.s0.P.y, S.s0.P.z, S.s1.P.x, S.s1.P.y, S.s1.P.z,
+                                          S.s2.P.x, S.s2.P.y, S.s2.P.z, S.s3.P.x, S.s3.P.y, S.s3.P.z,
+                                          __int_as_float(S.n)};
+                    if (saved) {
+                        bool same = __float_as_int(cyc[12 * BLOCK + tid]) == S.n;
+#pragma unroll
+                        for (int k = 0; k < 12; ++k)
+                            same = same && (k >= 3 * S.n || __float_as_int(cyc[k * BLOCK + tid]) == __float_as_int(st[k]));
+                        if (same) res = GJK_CAP;  // a repeated state: the cap is certain
+                    }
+                    ++lam;
+                    if (res == GJK_STILL && (!saved || lam == power)) {
+#pragma unroll
+                        for (int k = 0; k < 13; ++k) cyc[k * BLOCK + tid] = st[k];
+                        power = saved ? 2 * power : 1;
+                        saved = true;
+                        lam = 0;
+                    }
+                }
                 if (res != GJK_STILL || calls > np.max_gjk_iterations) {
                     const int g = res == GJK_STILL ? GJK_CAP : res;
                     src.gjk_result(c, g);
