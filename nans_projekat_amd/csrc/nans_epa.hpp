@@ -650,19 +650,18 @@ __device__ inline EpaResult epa_wave(EpaLds &L, const EpaShapes &sh, V3 p0, cons
 // ---------------------------------------------------------------------- four EPAs per wave
 // ResolveCollision for four pairs at once, one per 16-lane group (a row of the
 // wave): the same steps as epa_wave, with the polytope of each pair held by its
-// group -- faces f and 16 + f in lane f, vertices v and 16 + v in lane v (P, SupA,
-// SupB), P also in LDS for gathers -- group reductions by row-local DPP and group
-// broadcasts by ds_bpermute.  A polytope that outgrows a group (33 vertices, 33
-// faces, 17 horizon edges, or an edge key pushed twice) stops with EPA_SPILL and is
-// redone from its simplex by epa_wave, so the result is always epa_wave's.
+// group -- face f and vertex v in lane f / v (P, SupA, SupB; P also in LDS for
+// gathers) -- group reductions by row-local DPP and group broadcasts by
+// ds_bpermute.  A polytope that outgrows a group (17 vertices, 17 faces, 17
+// horizon edges, or an edge key pushed twice) stops with EPA_SPILL and is redone
+// from its simplex by epa_wave, so the result is always epa_wave's.
 constexpr int EPA_SPILL = 99;
-constexpr int EPG_MAXV = 32;
-constexpr int EPG_MAXF = 32;
+constexpr int EPG_MAX = 16;
 
 struct EpgLds {
-    float vp[4][EPG_MAXV][3];      // vertex P per group
-    float sf[4][EPG_MAXF][9];      // compaction staging: n, a, d, id, cl
-    int ekey[64];                  // horizon edges per group (16 each)
+    float vp[4][EPG_MAX][3];   // vertex P per group
+    float sf[4][EPG_MAX][9];   // compaction staging: n, a, d, id, cl
+    int ekey[64];              // horizon edges per group (16 each)
     int eids[64];
 };
 static_assert(sizeof(EpgLds) <= sizeof(EpaLds), "the grouped EPA reuses the wave's EPA LDS");
@@ -688,7 +687,6 @@ __device__ __forceinline__ float row_min(float v)
     v = fminf(v, dpp_f<0x140>(v));
     return v;
 }
-__device__ __forceinline__ int nth_set_bit32(uint32_t m, int n) { return nth_set_bit((uint64_t)m, n); }
 
 // The group's shapes: lanes 0-7 of each group hold the corners of A, 8-15 of B.
 __device__ inline EpaShapes epa_shapes_group(const Shape &A, const Shape &B)
@@ -735,6 +733,12 @@ __device__ inline Vtx group_support(const EpaShapes &s, V3 d)
     return v;
 }
 
+// classes (ca | cb << 8) of edge e of a face with packed classes cl
+__device__ __forceinline__ int edge_key(int cl, int e)
+{
+    return ((cl >> (8 * e)) & 255) | (((cl >> (8 * ((e + 1) % 3))) & 255) << 8);
+}
+
 // Every lane calls; lanes 0-3 of a group pass the supports of simplex vertex j in
 // sa0 / sb0.  act: the group has a pair.  Returns the group's result (EPA_NONE for
 // an idle group, EPA_SPILL to redo with epa_wave).
@@ -751,37 +755,37 @@ __device__ inline EpaResult epa_group(EpgLds &L, const EpaShapes &sh, V3 sa0, V3
     r.nverts = 4;
     bool run = act;
     const V3 z = mk3(0.0f, 0.0f, 0.0f);
-    // vertices v (slot 0) and 16 + v (slot 1) of lane v
-    V3 VA0 = j < 4 ? sa0 : z, VB0 = j < 4 ? sb0 : z;
-    V3 VP0 = j < 4 ? sub(sa0, sb0) : z;
-    V3 VA1 = z, VB1 = z, VP1 = z;
-    if (j < 4) st3(L.vp[g], j, VP0);
+    // vertex j of the group: P, SupA, SupB
+    V3 VA = j < 4 ? sa0 : z, VB = j < 4 ? sb0 : z;
+    V3 VP = j < 4 ? sub(sa0, sb0) : z;
+    if (j < 4) st3(L.vp[g], j, VP);
     int c0;
     {
         int cls = j;
         bool found = false;
 #pragma unroll
         for (int u = 0; u < 3; ++u) {
-            const V3 pu = gpull3(VP0, gb + u);
-            if (!found && u < j && veq(pu, VP0)) {
+            const V3 pu = gpull3(VP, gb + u);
+            if (!found && u < j && veq(pu, VP)) {
                 cls = u;
                 found = true;
             }
         }
-        c0 = has_nan(VP0) ? CLS_NAN : cls;
+        c0 = has_nan(VP) ? CLS_NAN : cls;
     }
-    FaceR F0 = FaceR{0, 0, 0, 0, 0, 0, 0, 0, 0}, F1 = F0;
+    FaceR F = FaceR{0, 0, 0, 0, 0, 0, 0, 0, 0};
     {
+        // ABC, ACD, ADB, BDC (code/nans.cpp:799-802)
         const int a = j == 3 ? 1 : 0;
         const int b = j == 0 ? 1 : (j == 1 ? 2 : 3);
         const int c = j == 0 ? 2 : (j == 1 ? 3 : (j == 2 ? 1 : 2));
-        const V3 Pa = gpull3(VP0, gb + a), Pb = gpull3(VP0, gb + b), Pc = gpull3(VP0, gb + c);
+        const V3 Pa = gpull3(VP, gb + a), Pb = gpull3(VP, gb + b), Pc = gpull3(VP, gb + c);
         const int ca = gpull_i(c0, gb + a), cb = gpull_i(c0, gb + b), cc = gpull_i(c0, gb + c);
-        if (j < 4) F0 = make_face(Pa, Pb, Pc, a, b, c, ca, cb, cc);
+        if (j < 4) F = make_face(Pa, Pb, Pc, a, b, c, ca, cb, cc);
     }
     int nf = 4, nv = 4, iter = 0;
     for (;;) {
-        // while (iter++ <= MAX_EPA_ITERATIONS) per group
+        // while (iter++ <= MAX_EPA_ITERATIONS), per group
         if (run && !(iter <= p.max_epa_iterations)) run = false;  // status stays EPA_ITER
         iter += 1;
         if (run) r.iters += 1;
@@ -790,34 +794,26 @@ __device__ inline EpaResult epa_group(EpgLds &L, const EpaShapes &sh, V3 sa0, V3
             run = false;
         }
         if (!ballot(run)) break;
-        // closest face
-        const float d0 = gpull_f(gabs(dot(mk3(F0.nx, F0.ny, F0.nz), mk3(F0.ax, F0.ay, F0.az))), gb);
-        const float k0 = (j < nf && F0.d < d0) ? F0.d : __builtin_inff();
-        const float k1 = (16 + j < nf && F1.d < d0) ? F1.d : __builtin_inff();
-        const float m = row_min(fminf(k0, k1));
-        const uint32_t b0 = gbits(ballot(k0 == m)), b1 = gbits(ballot(k1 == m));
-        int best = 0;
-        if (m != __builtin_inff()) best = b0 ? ffs64(b0) : (b1 ? 16 + ffs64(b1) : 0);
-        const int owner = gb + (best & 15);
-        const bool ks = best >= 16;
-        const V3 N = mk3(gpull_f(ks ? F1.nx : F0.nx, owner), gpull_f(ks ? F1.ny : F0.ny, owner),
-                         gpull_f(ks ? F1.nz : F0.nz, owner));
-        const float dsel = gpull_f(ks ? F1.d : F0.d, owner);
+        // closest face: the first face holding the minimum below face 0's distance
+        const float d0 = gpull_f(gabs(dot(mk3(F.nx, F.ny, F.nz), mk3(F.ax, F.ay, F.az))), gb);
+        const float k0 = (j < nf && F.d < d0) ? F.d : __builtin_inff();
+        const float m = row_min(k0);
+        const uint32_t b0 = gbits(ballot(k0 == m));
+        const int best = (m != __builtin_inff() && b0) ? ffs64(b0) : 0;
+        const int owner = gb + best;
+        const V3 N = mk3(gpull_f(F.nx, owner), gpull_f(F.ny, owner), gpull_f(F.nz, owner));
+        const float dsel = gpull_f(F.d, owner);
         const float cur = m == __builtin_inff() ? d0 : dsel;
-        const int fid = gpull_i(ks ? F1.id : F0.id, owner);
+        const int fid = gpull_i(F.id, owner);
         // new support along the stored normal
         const Vtx ns = group_support(sh, N);
         const bool conv = run && dot(N, ns.P) - cur < p.epa_tolerance;
         if (ballot(conv)) {
             // Barycentric (code/nans.cpp:771-785) of N * d on the chosen face
             const int a = fid & 255, b = (fid >> 8) & 255, c = (fid >> 16) & 255;
-            auto vsel = [&](V3 s0, V3 s1, int v) {
-                const bool hi = v >= 16;
-                return gpull3(mk3(hi ? s1.x : s0.x, hi ? s1.y : s0.y, hi ? s1.z : s0.z), gb + (v & 15));
-            };
-            const V3 PA = vsel(VP0, VP1, a), PB = vsel(VP0, VP1, b), PC = vsel(VP0, VP1, c);
-            const V3 AA = vsel(VA0, VA1, a), AB = vsel(VA0, VA1, b), AC = vsel(VA0, VA1, c);
-            const V3 BA = vsel(VB0, VB1, a), BB = vsel(VB0, VB1, b), BC = vsel(VB0, VB1, c);
+            const V3 PA = gpull3(VP, gb + (a & 15)), PB = gpull3(VP, gb + (b & 15)), PC = gpull3(VP, gb + (c & 15));
+            const V3 AA = gpull3(VA, gb + (a & 15)), AB = gpull3(VA, gb + (b & 15)), AC = gpull3(VA, gb + (c & 15));
+            const V3 BA = gpull3(VB, gb + (a & 15)), BB = gpull3(VB, gb + (b & 15)), BC = gpull3(VB, gb + (c & 15));
             if (conv) {
                 const V3 P = mul(N, cur);
                 const V3 e0 = sub(PB, PA), e1 = sub(PC, PA), e2 = sub(P, PA);
@@ -841,107 +837,86 @@ __device__ inline EpaResult epa_group(EpgLds &L, const EpaShapes &sh, V3 sa0, V3
                 run = false;
             }
         }
-        if (run && nv >= EPG_MAXV) {
+        if (run && nv >= EPG_MAX) {
             r.status = EPA_SPILL;
             run = false;
         }
         // append the vertex (its class: the first earlier vertex with the same P)
         const int vnew = nv;
-        int ncls;
-        {
-            const uint32_t m0 = gbits(ballot(j < nv && veq(VP0, ns.P)));
-            const uint32_t m1 = gbits(ballot(16 + j < nv && veq(VP1, ns.P)));
-            ncls = has_nan(ns.P) ? CLS_NAN : (m0 ? ffs64(m0) : (m1 ? 16 + ffs64(m1) : nv));
-        }
+        const uint32_t meq = gbits(ballot(j < nv && veq(VP, ns.P)));
+        const int ncls = has_nan(ns.P) ? CLS_NAN : (meq ? ffs64(meq) : nv);
         if (run) {
-            if (j == (vnew & 15)) {
-                if (vnew < 16) {
-                    VP0 = ns.P; VA0 = ns.A; VB0 = ns.B;
-                } else {
-                    VP1 = ns.P; VA1 = ns.A; VB1 = ns.B;
-                }
+            if (j == vnew) {
+                VP = ns.P;
+                VA = ns.A;
+                VB = ns.B;
             }
             if (j == 0) st3(L.vp[g], vnew, ns.P);
             nv += 1;
             r.nverts = nv;
         }
         // visibility
-        const bool w0 = run && j < nf && dot(mk3(F0.nx, F0.ny, F0.nz), sub(ns.P, mk3(F0.ax, F0.ay, F0.az))) > 0.0f;
-        const bool w1 = run && 16 + j < nf && dot(mk3(F1.nx, F1.ny, F1.nz), sub(ns.P, mk3(F1.ax, F1.ay, F1.az))) > 0.0f;
-        const uint32_t vis0 = gbits(ballot(w0)), vis1 = gbits(ballot(w1));
-        const uint32_t vis = vis0 | (vis1 << 16);
+        const bool w0 = run && j < nf && dot(mk3(F.nx, F.ny, F.nz), sub(ns.P, mk3(F.ax, F.ay, F.az))) > 0.0f;
+        const uint32_t vis = gbits(ballot(w0));
         const int nvis = __popc(vis);
         // horizon: edges of the visible faces in face order; an edge survives iff its
-        // reverse is not among them (no key may occur twice, else spill)
-        int key[2][3], rev[2][3], nsame[2][3], nrev[2][3];
-#pragma unroll
-        for (int sl = 0; sl < 2; ++sl) {
-            const int cl = sl ? F1.cl : F0.cl;
-#pragma unroll
-            for (int e = 0; e < 3; ++e) {
-                const int ca = (cl >> (8 * e)) & 255, cb = (cl >> (8 * ((e + 1) % 3))) & 255;
-                key[sl][e] = ca | (cb << 8);
-                rev[sl][e] = (ca == CLS_NAN || cb == CLS_NAN) ? -1 : (cb | (ca << 8));
-                nsame[sl][e] = 0;
-                nrev[sl][e] = 0;
-            }
-        }
+        // reverse is not among them, when no key is pushed twice (else spill).  Bits
+        // of this lane's three edges: seen / twice for same-key and reverse matches.
+        const int cl = F.cl;
+        uint32_t s1 = 0, s2 = 0, r1 = 0, r2 = 0;
         const int tmax = max(max(readlane_i(nvis, 0), readlane_i(nvis, 16)), max(readlane_i(nvis, 32), readlane_i(nvis, 48)));
+        uint32_t rest = vis;
         for (int t = 0; t < tmax; ++t) {
-            const bool has = t < nvis;
-            const int f = has ? nth_set_bit32(vis, t) : 0;
-            const int co = gpull_i(f >= 16 ? F1.cl : F0.cl, gb + (f & 15));
-            if (has) {
+            const int f = rest ? ffs64(rest) : 0;
+            rest &= rest - 1u;
+            const int co = gpull_i(cl, gb + f);
+            if (t < nvis) {
 #pragma unroll
                 for (int u = 0; u < 3; ++u) {
-                    const int kt = ((co >> (8 * u)) & 255) | (((co >> (8 * ((u + 1) % 3))) & 255) << 8);
+                    const int kt = edge_key(co, u);
 #pragma unroll
-                    for (int sl = 0; sl < 2; ++sl) {
-#pragma unroll
-                        for (int e = 0; e < 3; ++e) {
-                            nsame[sl][e] += kt == key[sl][e];
-                            nrev[sl][e] += kt == rev[sl][e];
+                    for (int e = 0; e < 3; ++e) {
+                        const int ke = edge_key(cl, e);
+                        const int ca = ke & 255, cb = ke >> 8;
+                        const bool nan = ca == CLS_NAN || cb == CLS_NAN;
+                        const uint32_t bit = 1u << e;
+                        if (kt == ke) {
+                            s2 |= s1 & bit;
+                            s1 |= bit;
+                        }
+                        if (!nan && kt == (cb | (ca << 8))) {
+                            r2 |= r1 & bit;
+                            r1 |= bit;
                         }
                     }
                 }
             }
         }
-        bool dup = false;
-        int keep[2] = {0, 0};
+        uint32_t nanbits = 0;
 #pragma unroll
-        for (int sl = 0; sl < 2; ++sl) {
-            const bool mine = sl ? w1 : w0;
-#pragma unroll
-            for (int e = 0; e < 3; ++e) {
-                dup |= mine && rev[sl][e] >= 0 && (nsame[sl][e] > 1 || nrev[sl][e] > 1);
-                keep[sl] |= (mine && (rev[sl][e] < 0 || nrev[sl][e] == 0)) ? (1 << e) : 0;
-            }
+        for (int e = 0; e < 3; ++e) {
+            const int ke = edge_key(cl, e);
+            nanbits |= ((ke & 255) == CLS_NAN || (ke >> 8) == CLS_NAN) ? (1u << e) : 0u;
         }
-        const int cnt0 = __popc(keep[0]), cnt1 = __popc(keep[1]);
+        const bool dup = w0 && ((s2 | r2) & ~nanbits & 7u) != 0u;
+        const int keep = w0 ? (int)(~r1 & 7u) : 0;   // NaN edges never match: r1 has no NaN bits
+        const int cnt = __popc(keep);
         const uint32_t lower = (1u << j) - 1u;
-        const uint32_t p00 = gbits(ballot(cnt0 & 1)), p01 = gbits(ballot(cnt0 & 2));
-        const uint32_t p10 = gbits(ballot(cnt1 & 1)), p11 = gbits(ballot(cnt1 & 2));
-        const int tot0 = __popc(p00) + 2 * __popc(p01);
-        const int ne = tot0 + __popc(p10) + 2 * __popc(p11);
-        if (run && (gbits(ballot(dup)) != 0u || ne > 16)) {
+        const uint32_t p0 = gbits(ballot(cnt & 1)), p1 = gbits(ballot(cnt & 2));
+        const int ne = __popc(p0) + 2 * __popc(p1);
+        if (run && (gbits(ballot(dup)) != 0u || ne > EPG_MAX)) {
             r.status = EPA_SPILL;
             run = false;
         }
-        if (run) {
-            int o0 = __popc(p00 & lower) + 2 * __popc(p01 & lower);
-            int o1 = tot0 + __popc(p10 & lower) + 2 * __popc(p11 & lower);
+        if (run && keep) {
+            int o = __popc(p0 & lower) + 2 * __popc(p1 & lower);
+            const int id = F.id;
 #pragma unroll
-            for (int sl = 0; sl < 2; ++sl) {
-                const int id = sl ? F1.id : F0.id;
-                int &o = sl ? o1 : o0;
-#pragma unroll
-                for (int e = 0; e < 3; ++e) {
-                    if ((keep[sl] >> e) & 1) {
-                        const int s0 = 8 * e, s1 = 8 * ((e + 1) % 3);
-                        L.eids[gb + o] = ((id >> s0) & 255) | (((id >> s1) & 255) << 8);
-                        L.ekey[gb + o] = key[sl][e];
-                        ++o;
-                    }
+            for (int e = 0; e < 3; ++e) {
+                if ((keep >> e) & 1) {
+                    L.eids[gb + o] = edge_key(id, e);
+                    L.ekey[gb + o] = edge_key(cl, e);
+                    ++o;
                 }
             }
         }
@@ -950,15 +925,15 @@ __device__ inline EpaResult epa_group(EpgLds &L, const EpaShapes &sh, V3 sa0, V3
         if (run && j < ne) {
             const int k2 = L.ekey[gb + j], i2 = L.eids[gb + j];
             ea = i2 & 255;
-            eb = (i2 >> 8) & 255;
+            eb = i2 >> 8;
             eca = k2 & 255;
-            ecb = (k2 >> 8) & 255;
+            ecb = k2 >> 8;
         }
         // ordered removal of the visible faces, then the new ones, through LDS
-        const bool al0 = j < nf && !w0, al1 = 16 + j < nf && !w1;
-        const uint32_t alive = gbits(ballot(al0)) | (gbits(ballot(al1)) << 16);
+        const bool al = j < nf && !w0;
+        const uint32_t alive = gbits(ballot(al));
         const int nsurv = __popc(alive);
-        if (run && nsurv + ne > EPG_MAXF) {
+        if (run && nsurv + ne > EPG_MAX) {
             r.status = EPA_SPILL;
             run = false;
         }
@@ -973,23 +948,20 @@ __device__ inline EpaResult epa_group(EpgLds &L, const EpaShapes &sh, V3 sa0, V3
             d[8] = __int_as_float(f.cl);
         };
         if (run) {
-            if (al0) stage(__popc(alive & lower), F0);
-            if (al1) stage(__popc(alive & ((lower << 16) | 0xffffu)), F1);
+            if (al) stage(__popc(alive & lower), F);
             if (j < ne) stage(nsurv + j, gn);
         }
         wave_sync();
         if (run) {
             const int nfn = nsurv + ne;
-            auto unstage = [&](int at, FaceR &f) {
-                const float *d = L.sf[g][at];
-                f.nx = d[0]; f.ny = d[1]; f.nz = d[2];
-                f.ax = d[3]; f.ay = d[4]; f.az = d[5];
-                f.d = d[6];
-                f.id = __float_as_int(d[7]);
-                f.cl = __float_as_int(d[8]);
-            };
-            if (j < nfn) unstage(j, F0);
-            if (16 + j < nfn) unstage(16 + j, F1);
+            if (j < nfn) {
+                const float *d = L.sf[g][j];
+                F.nx = d[0]; F.ny = d[1]; F.nz = d[2];
+                F.ax = d[3]; F.ay = d[4]; F.az = d[5];
+                F.d = d[6];
+                F.id = __float_as_int(d[7]);
+                F.cl = __float_as_int(d[8]);
+            }
             nf = nfn;
             if (nf > r.max_faces) r.max_faces = nf;
         }

@@ -136,36 +136,72 @@ struct WorldSrc {
 };
 
 // Dependency rounds of SolveConstraints over con[0..ncon) (see the file comment).
-// head[] has room for n ints; done[] for ncon.
+// head[] has room for n ints; done[] for ncon.  The body slots of each thread's
+// first SOLVE_REG contacts stay in registers for all rounds; contacts beyond
+// SOLVE_REG * BLOCK are read from con[] each round.
+constexpr int SOLVE_REG = 2;
+
 template <int BLOCK>
 __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int ncon, int *done, int *head,
                                     const FloorDev &fl, const StepConsts &k, float Cf, float Cr)
 {
     const int tid = threadIdx.x;
-    for (int c = tid; c < ncon; c += BLOCK) done[c] = 0;
+    int sa[SOLVE_REG], sb[SOLVE_REG];
+    bool pend[SOLVE_REG];
+#pragma unroll
+    for (int r = 0; r < SOLVE_REG; ++r) {
+        const int c = tid + r * BLOCK;
+        pend[r] = c < ncon;
+        sa[r] = pend[r] ? con[c].slot_a : 0;
+        sb[r] = pend[r] ? con[c].slot_b : -1;
+    }
+    for (int c = tid + SOLVE_REG * BLOCK; c < ncon; c += BLOCK) done[c] = 0;
     __syncthreads();
     int pending = ncon;
     while (pending > 0) {
         for (int i = tid; i < n; i += BLOCK) head[i] = INT_MAX;
         __syncthreads();
-        for (int c = tid; c < ncon; c += BLOCK) {
+#pragma unroll
+        for (int r = 0; r < SOLVE_REG; ++r) {
+            if (pend[r]) {
+                const int c = tid + r * BLOCK;
+                atomicMin(&head[sa[r]], c);
+                if (sb[r] >= 0) atomicMin(&head[sb[r]], c);
+            }
+        }
+        for (int c = tid + SOLVE_REG * BLOCK; c < ncon; c += BLOCK) {
             if (!done[c]) {
                 atomicMin(&head[con[c].slot_a], c);
-                int sb = con[c].slot_b;
-                if (sb >= 0) atomicMin(&head[sb], c);
+                const int b = con[c].slot_b;
+                if (b >= 0) atomicMin(&head[b], c);
             }
         }
         __syncthreads();
         int left = 0;
-        for (int c = tid; c < ncon; c += BLOCK) {
+        auto apply = [&](int c) {
+            ContactRec r = con[c];
+            constraint(B, r, fl, k, Cf, Cr);
+            con[c].dln = r.dln;
+            con[c].dlt1 = r.dlt1;
+            con[c].dlt2 = r.dlt2;
+        };
+#pragma unroll
+        for (int r = 0; r < SOLVE_REG; ++r) {
+            if (pend[r]) {
+                const int c = tid + r * BLOCK;
+                if (head[sa[r]] == c && (sb[r] < 0 || head[sb[r]] == c)) {
+                    apply(c);
+                    pend[r] = false;
+                } else {
+                    left = 1;
+                }
+            }
+        }
+        for (int c = tid + SOLVE_REG * BLOCK; c < ncon; c += BLOCK) {
             if (!done[c]) {
-                int sa = con[c].slot_a, sb = con[c].slot_b;
-                if (head[sa] == c && (sb < 0 || head[sb] == c)) {
-                    ContactRec r = con[c];
-                    constraint(B, r, fl, k, Cf, Cr);
-                    con[c].dln = r.dln;
-                    con[c].dlt1 = r.dlt1;
-                    con[c].dlt2 = r.dlt2;
+                const int a = con[c].slot_a, b = con[c].slot_b;
+                if (head[a] == c && (b < 0 || head[b] == c)) {
+                    apply(c);
                     done[c] = 1;
                 } else {
                     left = 1;
@@ -412,27 +448,50 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgs a)
             bradL[i] = i < nb ? box_vertices_body(Lb, i, vertsL + 24 * i) : Lb.at(F_SX, i);
         }
         __syncthreads();
-        // cull over the canonical pairs, ordered
+        // cull over the canonical pairs, in canonical order: wave w takes the w-th
+        // contiguous share of the list, with no workgroup barrier, and appends the
+        // survivors to its own region of the contact-record scratch (idle until the
+        // narrowphase); the regions are then concatenated into cand[] in wave order
         const int64_t npairs = pair_count(nb, ns);
-        int ncand = 0;
-        for (int64_t base = 0; base < npairs; base += BLOCK) {
-            int64_t p = base + tid;
-            bool keep = false;
-            if (p < npairs) {
-                PairId q = decode_pair(p, nb, ns);
+        const int wv = tid >> 6, lane = tid & 63;
+        const int64_t share = (npairs + NW - 1) / NW;
+        const int64_t q0 = min(npairs, (int64_t)wv * share), q1 = min(npairs, q0 + share);
+        int *wbuf = reinterpret_cast<int *>(rec) + (int64_t)wv * a.cap;  // cap ints per wave
+        int mine = 0;
+        for (int64_t b = q0; b < q1; b += 64) {
+            const int p = (int)(b + lane);
+            bool k = false;
+            if (p < q1) {
+                const PairId q = decode_pair32(p, nb, ns);
                 int sa, sb;
                 pair_slots(q, nb, sa, sb);
                 V3 ca, cb;
                 float ra, rb;
                 slot_sphere(Lb, bradL, sa, fl, ca, ra);
                 slot_sphere(Lb, bradL, sb, fl, cb, rb);
-                keep = cull_keep(ca, ra, cb, rb);
+                k = cull_keep(ca, ra, cb, rb);
             }
-            int tot;
-            int r = block_rank<BLOCK>(keep, wave_tot, tot);
-            if (keep && ncand + r < a.cap) cand[ncand + r] = (int)p;
-            ncand += tot;
+            const uint64_t m = ballot(k);
+            const int r = mine + __popcll(m & ((1ull << lane) - 1ull));
+            if (k && r < a.cap) wbuf[r] = p;
+            mine += __popcll(m);
         }
+        if (lane == 0) wave_tot[wv] = mine;
+        __syncthreads();
+        int ncand = 0;
+        int woff[NW];
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            woff[i] = ncand;
+            ncand += wave_tot[i];
+        }
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            const int cnt = min(wave_tot[i], a.cap);
+            const int *src = reinterpret_cast<const int *>(rec) + (int64_t)i * a.cap;
+            for (int k = tid; k < cnt && woff[i] + k < a.cap; k += BLOCK) cand[woff[i] + k] = src[k];
+        }
+        __syncthreads();
         if (ncand > a.cap) {
             if (tid == 0) atomicAdd(&a.cnt[CNT_CAND_OVERFLOW], 1ull);
             ncand = a.cap;

@@ -133,8 +133,9 @@ __host__ __device__ inline int64_t pair_count(int nb, int ns)
 // Row i of the upper triangle (i < j) holding linear index p; row i starts at i*(2n-i-1)/2.
 __device__ inline void tri_decode(int64_t p, int n, int &i, int &j)
 {
-    double b = 2.0 * n - 1.0;
-    int r = (int)((b - sqrt(b * b - 8.0 * (double)p)) * 0.5);
+    // a binary32 estimate of the row; the two loops below make it exact
+    const float b = 2.0f * (float)n - 1.0f;
+    int r = (int)((b - __builtin_amdgcn_sqrtf(fmaxf(b * b - 8.0f * (float)p, 0.0f))) * 0.5f);
     if (r < 0) r = 0;
     auto start = [n](int64_t row) { return row * (2 * (int64_t)n - row - 1) / 2; };
     while (r + 1 < n && start(r + 1) <= p) ++r;
@@ -175,6 +176,58 @@ __device__ inline PairId decode_pair(int64_t p, int nb, int ns)
     p -= (int64_t)nb * ns;
     q.type = NANS_SS;
     tri_decode(p, ns, q.i, q.j);
+    return q;
+}
+
+// decode_pair for worlds whose pair count fits 24 bits (the batched path):
+// binary32 estimates of row and quotient made exact by integer fix-ups.
+__host__ __device__ inline void tri_decode32(int p, int n, int &i, int &j)
+{
+    const float b = 2.0f * (float)n - 1.0f;
+    const float disc = b * b - 8.0f * (float)p;
+    int r = (int)((b - sqrtf(disc > 0.0f ? disc : 0.0f)) * 0.5f);
+    r = r < 0 ? 0 : (r > n - 1 ? n - 1 : r);
+    auto start = [n](int row) { return row * (2 * n - row - 1) / 2; };
+    while (r + 1 < n && start(r + 1) <= p) ++r;
+    while (r > 0 && start(r) > p) --r;
+    i = r;
+    j = p - start(r) + r + 1;
+}
+
+__host__ __device__ inline PairId decode_pair32(int p, int nb, int ns)
+{
+    PairId q;
+    const int ncc = nb * (nb - 1) / 2;
+    if (p < ncc) {
+        q.type = NANS_CC;
+        tri_decode32(p, nb, q.i, q.j);
+        return q;
+    }
+    p -= ncc;
+    if (p < nb) {
+        q.type = NANS_CF;
+        q.i = q.j = p;
+        return q;
+    }
+    p -= nb;
+    if (p < ns) {
+        q.type = NANS_SF;
+        q.i = q.j = p;
+        return q;
+    }
+    p -= ns;
+    if (p < nb * ns) {
+        q.type = NANS_CS;
+        int i = (int)((float)p * (1.0f / (float)ns));
+        while (i * ns > p) --i;
+        while ((i + 1) * ns <= p) ++i;
+        q.i = i;
+        q.j = p - i * ns;
+        return q;
+    }
+    p -= nb * ns;
+    q.type = NANS_SS;
+    tri_decode32(p, ns, q.i, q.j);
     return q;
 }
 
