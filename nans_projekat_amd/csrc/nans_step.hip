@@ -141,11 +141,17 @@ struct WorldSrc {
 // SOLVE_REG * BLOCK are read from con[] each round.
 constexpr int SOLVE_REG = 2;
 
+// pre[0..npre) (may be empty) receives Constraint's position-only terms of the
+// first npre contacts in one parallel pass, so that the rounds only evaluate the
+// velocity-dependent part for them.
 template <int BLOCK>
 __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int ncon, int *done, int *head,
-                                    const FloorDev &fl, const StepConsts &k, float Cf, float Cr)
+                                    const FloorDev &fl, const StepConsts &k, float Cf, float Cr,
+                                    ContactTerms *pre = nullptr, int npre = 0)
 {
     const int tid = threadIdx.x;
+    npre = min(npre, ncon);
+    for (int c = tid; c < npre; c += BLOCK) contact_terms(B, con[c], fl, k, pre[c]);
     int sa[SOLVE_REG], sb[SOLVE_REG];
     bool pend[SOLVE_REG];
 #pragma unroll
@@ -180,7 +186,8 @@ __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int
         int left = 0;
         auto apply = [&](int c) {
             ContactRec r = con[c];
-            constraint(B, r, fl, k, Cf, Cr);
+            if (c < npre) contact_apply(B, r, pre[c], k, Cf, Cr);
+            else constraint(B, r, fl, k, Cf, Cr);
             con[c].dln = r.dln;
             con[c].dlt1 = r.dlt1;
             con[c].dlt2 = r.dlt2;
@@ -521,7 +528,8 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgs a)
         __syncthreads();
         const float Cf = a.wfric ? a.wfric[w] : a.k.friction;
         const float Cr = a.wrest ? a.wrest[w] : a.k.restitution;
-        solve_rounds<BLOCK>(Lb, n, con, ncon, done, headL, fl, a.k, Cf, Cr);
+        solve_rounds<BLOCK>(Lb, n, con, ncon, done, headL, fl, a.k, Cf, Cr, reinterpret_cast<ContactTerms *>(epa),
+                            (int)(NW * sizeof(EpaLds) / sizeof(ContactTerms)));
         // IntegrateVelocities and write-back
         for (int i = tid; i < n; i += BLOCK) {
             integrate_velocities_body(Lb, i, a.k.dt);

@@ -289,38 +289,67 @@ __host__ __device__ inline BodyRef body_ref(const Planes &B, int slot, const Flo
     return r;
 }
 
-// Solve one contact against the current V, W of its bodies and apply the impulses.
-// friction / restitution are the pair's coefficients.
-__host__ __device__ inline void constraint(const Planes &B, ContactRec &c, const FloorDev &fl, const StepConsts &k,
-                                  float Cf, float Cr)
+// Constraint's terms that depend only on positions, masses and the contact
+// (code/nans.cpp:1115-1170): they do not change during SolveConstraints.
+struct ContactTerms {
+    V3 N, T1, T2, RN1, RN2, R1T1, R2T1, R1T2, R2T2;
+    float JMJn, JMJt1, JMJt2;
+    float bdepth;  // -beta / dt * Depth
+    float InvM1, InvM2, InvI1, InvI2;
+};
+
+__host__ __device__ inline void contact_terms(const Planes &B, const ContactRec &c, const FloorDev &fl,
+                                              const StepConsts &k, ContactTerms &t)
 {
     BodyRef A = body_ref(B, c.slot_a, fl);
     BodyRef Bo = body_ref(B, c.slot_b, fl);
     V3 N = normalize(mk3(c.n[0], c.n[1], c.n[2]));
-    V3 PosA = A.P, PosB = Bo.P;
-    float InvI1 = 1.0f / A.moi, InvI2 = 1.0f / Bo.moi;
-    float InvM1 = 1.0f / A.mass, InvM2 = 1.0f / Bo.mass;
-    V3 V1 = A.V, V2 = Bo.V, W1 = A.W, W2 = Bo.W;
+    const V3 PosA = A.P, PosB = Bo.P;
+    t.InvI1 = 1.0f / A.moi;
+    t.InvI2 = 1.0f / Bo.moi;
+    t.InvM1 = 1.0f / A.mass;
+    t.InvM2 = 1.0f / Bo.mass;
     if (veq(N, mk3(0.0f, 0.0f, 0.0f))) N = normalize(sub(PosB, PosA));
-    V3 R1 = sub(mk3(c.pa[0], c.pa[1], c.pa[2]), PosA);
-    V3 R2 = sub(mk3(c.pb[0], c.pb[1], c.pb[2]), PosB);
-    V3 T1 = N.x >= 0.57735f ? mk3(N.y, -N.x, 0.0f) : mk3(0.0f, N.z, -N.y);
-    V3 T2 = cross(N, T1);
-    float Depth = dot(sub(add(PosA, R1), add(PosB, R2)), N);
-    V3 RN1 = cross(R1, N), RN2 = cross(R2, N);
-    float JMJn = InvM1 + InvM2;
-    JMJn += InvI1 * dot(RN1, RN1) - InvI2 * dot(neg(RN2), neg(RN2));
-    JMJn = 1.0f / JMJn;
+    const V3 R1 = sub(mk3(c.pa[0], c.pa[1], c.pa[2]), PosA);
+    const V3 R2 = sub(mk3(c.pb[0], c.pb[1], c.pb[2]), PosB);
+    const V3 T1 = N.x >= 0.57735f ? mk3(N.y, -N.x, 0.0f) : mk3(0.0f, N.z, -N.y);
+    const V3 T2 = cross(N, T1);
+    const float Depth = dot(sub(add(PosA, R1), add(PosB, R2)), N);
+    t.N = N;
+    t.T1 = T1;
+    t.T2 = T2;
+    t.RN1 = cross(R1, N);
+    t.RN2 = cross(R2, N);
+    float JMJn = t.InvM1 + t.InvM2;
+    JMJn += t.InvI1 * dot(t.RN1, t.RN1) - t.InvI2 * dot(neg(t.RN2), neg(t.RN2));
+    t.JMJn = 1.0f / JMJn;
+    t.bdepth = -k.beta / k.dt * Depth;
+    t.R1T1 = cross(R1, T1);
+    t.R2T1 = cross(R2, T1);
+    t.R1T2 = cross(R1, T2);
+    t.R2T2 = cross(R2, T2);
+    float JMJt1 = t.InvM1 + t.InvM2;
+    JMJt1 += t.InvI1 * dot(t.R1T1, t.R1T1) - t.InvI2 * dot(neg(t.R2T1), neg(t.R2T1));
+    t.JMJt1 = 1.0f / JMJt1;
+    float JMJt2 = t.InvM1 + t.InvM2;
+    JMJt2 += t.InvI1 * dot(t.R1T2, t.R1T2) - t.InvI2 * dot(neg(t.R2T2), neg(t.R2T2));
+    t.JMJt2 = 1.0f / JMJt2;
+}
+
+// The rest of Constraint (code/nans.cpp:1148-1329): relative velocities from the
+// current V, W of the bodies, the impulse iterations, and the application.
+// friction / restitution are the pair's coefficients.
+__host__ __device__ inline void contact_apply(const Planes &B, ContactRec &c, const ContactTerms &t,
+                                              const StepConsts &k, float Cf, float Cr)
+{
+    const V3 zero = mk3(0.0f, 0.0f, 0.0f);
+    const V3 V1 = B.get3(F_VX, c.slot_a), W1 = B.get3(F_WX, c.slot_a);
+    const V3 V2 = c.slot_b >= 0 ? B.get3(F_VX, c.slot_b) : zero;
+    const V3 W2 = c.slot_b >= 0 ? B.get3(F_WX, c.slot_b) : zero;
+    const V3 N = t.N, T1 = t.T1, T2 = t.T2;
     V3 dVn = sub(sub(add(V1, cross(W1, N)), V2), cross(W2, N));
     float JdVn = dot(dVn, N);
-    float Bb = -k.beta / k.dt * Depth + Cr * JdVn;
-    V3 R1T1 = cross(R1, T1), R2T1 = cross(R2, T1), R1T2 = cross(R1, T2), R2T2 = cross(R2, T2);
-    float JMJt1 = InvM1 + InvM2;
-    JMJt1 += InvI1 * dot(R1T1, R1T1) - InvI2 * dot(neg(R2T1), neg(R2T1));
-    JMJt1 = 1.0f / JMJt1;
-    float JMJt2 = InvM1 + InvM2;
-    JMJt2 += InvI1 * dot(R1T2, R1T2) - InvI2 * dot(neg(R2T2), neg(R2T2));
-    JMJt2 = 1.0f / JMJt2;
+    float Bb = t.bdepth + Cr * JdVn;
     V3 dVt1 = sub(sub(add(V1, cross(W1, T1)), V2), cross(W2, T1));
     float JdVt1 = dot(dVt1, T1);
     V3 dVt2 = sub(sub(add(V1, cross(W1, T2)), V2), cross(W2, T2));
@@ -329,19 +358,19 @@ __host__ __device__ inline void constraint(const Planes &B, ContactRec &c, const
     float sumN = 0.0f, sumT1 = 0.0f, sumT2 = 0.0f, dln = 0.0f, dlt1 = 0.0f, dlt2 = 0.0f;
     const double cf_sqrt2 = 1.4142135623730951 * (double)Cf;
     for (int it = 0; it < k.solver_iterations; ++it) {
-        float LambdaN = (-JdVn + Bb) * JMJn;
+        float LambdaN = (-JdVn + Bb) * t.JMJn;
         float old = sumN;
         sumN += LambdaN;
         if (sumN < 0.0f) sumN = 0.0f;
         dln = sumN - old;
         float MaxLT = (float)(cf_sqrt2 * (double)sumN);
-        float LambdaT1 = (-JdVt1) * JMJt1;
+        float LambdaT1 = (-JdVt1) * t.JMJt1;
         float oldT1 = sumT1;
         sumT1 += LambdaT1;
         if (sumT1 < -MaxLT) sumT1 = -MaxLT;
         if (sumT1 > MaxLT) sumT1 = MaxLT;
         dlt1 = sumT1 - oldT1;
-        float LambdaT2 = (-JdVt2) * JMJt2;
+        float LambdaT2 = (-JdVt2) * t.JMJt2;
         float oldT2 = sumT2;
         sumT2 += LambdaT2;
         if (sumT2 < -MaxLT) sumT2 = -MaxLT;
@@ -351,10 +380,11 @@ __host__ __device__ inline void constraint(const Planes &B, ContactRec &c, const
     c.dln = dln;
     c.dlt1 = dlt1;
     c.dlt2 = dlt2;
+    const float InvM1 = t.InvM1, InvM2 = t.InvM2, InvI1 = t.InvI1, InvI2 = t.InvI2;
     V3 LinN = mul(N, dln), LinT1 = mul(T1, dlt1), LinT2 = mul(T2, dlt2);
-    V3 AngI1 = mul(RN1, dln), AngI2 = mul(RN2, dln);
-    V3 AngI1T1 = mul(R1T1, dlt1), AngI2T1 = mul(R2T1, dlt1);
-    V3 AngI1T2 = mul(R1T2, dlt2), AngI2T2 = mul(R2T2, dlt2);
+    V3 AngI1 = mul(t.RN1, dln), AngI2 = mul(t.RN2, dln);
+    V3 AngI1T1 = mul(t.R1T1, dlt1), AngI2T1 = mul(t.R2T1, dlt1);
+    V3 AngI1T2 = mul(t.R1T2, dlt2), AngI2T2 = mul(t.R2T2, dlt2);
     V3 va = V1, wa = W1, vb = V2, wb = W2;
     va = add(va, mul(LinN, InvM1));
     vb = sub(vb, mul(LinN, InvM2));
@@ -374,6 +404,16 @@ __host__ __device__ inline void constraint(const Planes &B, ContactRec &c, const
         B.set3(F_VX, c.slot_b, vb);
         B.set3(F_WX, c.slot_b, wb);
     }
+}
+
+// Solve one contact against the current V, W of its bodies and apply the impulses
+// (code/nans.cpp:1020-1329).
+__host__ __device__ inline void constraint(const Planes &B, ContactRec &c, const FloorDev &fl, const StepConsts &k,
+                                           float Cf, float Cr)
+{
+    ContactTerms t;
+    contact_terms(B, c, fl, k, t);
+    contact_apply(B, c, t, k, Cf, Cr);
 }
 
 }  // namespace nans
