@@ -18,6 +18,8 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnans_gpu.so")
+DRIVER_SRC = os.path.join(HERE, "driver", "nans_headless.cpp")
+DRIVER = os.path.join(HERE, "nans_headless")
 OBJ_DIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -51,9 +53,19 @@ def build(verbose: bool = False) -> str:
     if res.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), res.stderr))
     os.replace(tmp, LIB)
+    build_driver()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", DRIVER)
     return LIB
+
+
+def build_driver() -> str:
+    """The headless driver (host C++ only; it loads libnans_gpu.so with dlopen)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", DRIVER_SRC, "-ldl", "-o", DRIVER]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("driver build failed:\n%s\n%s" % (" ".join(cmd), res.stderr))
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -353,3 +353,42 @@ def config5(seed: int = SEED, nsteps: int = 480, nx: int = 32, ny: int = 8, nz: 
 def pair_counts(nb: int, ns: int):
     """Canonical pair counts per kind (code/nans.cpp:1355-1535): CC, CF, SF, CS, SS."""
     return nb * (nb - 1) // 2, nb, ns, nb * ns, ns * (ns - 1) // 2
+
+
+# ---------------------------------------------------------------------- files of the headless driver
+def save_scene(scene: Scene, path: str) -> None:
+    """Write a scene in the text format nans_projekat_amd/driver/nans_headless.cpp
+    reads ("nans-scene 1"); binary32 values are written with "%.9g", which reads back
+    exactly.  Force and torque are not part of the format (zero)."""
+    f9 = lambda x: "%.9g" % float(np.float32(x))
+    p = scene.params
+    with open(path, "w") as f:
+        f.write("nans-scene 1\n")
+        f.write("params %d %s %s %s %s %s %s %d %d %s %d %s %s %s %s\n" % (
+            p.solver_iterations, f9(p.beta), f9(p.restitution), f9(p.friction), f9(p.damping), f9(p.gravity),
+            f9(p.dt), p.max_gjk_iterations, p.max_epa_iterations, f9(p.epa_tolerance), p.fp64_islands,
+            " ".join(f9(x) for x in p.floor_pos), " ".join(f9(x) for x in p.floor_scale), f9(p.floor_mass),
+            f9(p.floor_moi)))
+        f.write("worlds %d\n" % scene.nworlds)
+        for w in range(scene.nworlds):
+            a, b = int(scene.offsets[w]), int(scene.offsets[w + 1])
+            fr = "-" if scene.world_friction is None else f9(scene.world_friction[w])
+            rs = "-" if scene.world_restitution is None else f9(scene.world_restitution[w])
+            f.write("world %d %s %s\n" % (b - a, fr, rs))
+            for i in range(a, b):
+                vals = [*scene.pos[i], *scene.vel[i], *scene.angvel[i], *scene.angles[i], *scene.scale[i],
+                        scene.mass[i], scene.moi[i]]
+                f.write("body %s %s\n" % ("box" if scene.kind[i] == BOX else "sphere", " ".join(f9(v) for v in vals)))
+
+
+def load_trajectory(path: str) -> dict:
+    """Read a trajectory written by nans_headless: {"steps": [nframes], "pos":
+    [nframes][nbodies][3], "angles": [nframes][nbodies][3], "steps_per_frame": k}."""
+    with open(path, "rb") as f:
+        if f.read(8) != b"NANSTRJ1":
+            raise ValueError("%s is not a NANSTRJ1 trajectory" % path)
+        n, nframes, per = np.frombuffer(f.read(12), "<i4")
+        rec = np.dtype([("step", "<i4"), ("pos", "<f4", (n, 3)), ("angles", "<f4", (n, 3))])
+        fr = np.frombuffer(f.read(), rec, count=nframes)
+    return {"steps": fr["step"].copy(), "pos": fr["pos"].copy(), "angles": fr["angles"].copy(),
+            "steps_per_frame": int(per)}
