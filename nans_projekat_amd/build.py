@@ -20,6 +20,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnans_gpu.so")
 DRIVER_SRC = os.path.join(HERE, "driver", "nans_headless.cpp")
 DRIVER = os.path.join(HERE, "nans_headless")
+SIM_SRC = os.path.join(HERE, "driver", "nans_sim.cpp")
+SIM = os.path.join(HERE, "libnans_sim.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -55,16 +57,22 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     build_driver()
     if verbose:
-        print("built", LIB, "and", DRIVER)
+        print("built", LIB, DRIVER, "and", SIM)
     return LIB
 
 
 def build_driver() -> str:
-    """The headless driver (host C++ only; it loads libnans_gpu.so with dlopen)."""
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", DRIVER_SRC, "-ldl", "-o", DRIVER]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError("driver build failed:\n%s\n%s" % (" ".join(cmd), res.stderr))
+    """The headless driver and the SimUpdateAndRender adapter libnans_sim.so (host
+    C++ only; both load libnans_gpu.so with dlopen).  The adapter's camera and
+    input arithmetic is binary32 without contraction, like the reference's."""
+    cmds = [["g++", "-O2", "-std=c++17", "-Wall", DRIVER_SRC, "-ldl", "-o", DRIVER],
+            ["g++", "-O2", "-std=c++17", "-Wall", "-ffp-contract=off", "-shared", "-fPIC", SIM_SRC, "-ldl", "-lm",
+             "-o", SIM + ".tmp"]]
+    for cmd in cmds:
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError("driver build failed:\n%s\n%s" % (" ".join(cmd), res.stderr))
+    os.replace(SIM + ".tmp", SIM)
     return DRIVER
 
 

@@ -7,16 +7,7 @@
 // (drop-in) or the batched step, and writes a trajectory file for comparison and
 // replay.
 //
-// Scene file (text, '#' starts a comment line; floats as written by "%.9g" read
-// back exactly):
-//   nans-scene 1
-//   params <solver_iterations> <beta> <restitution> <friction> <damping> <gravity> <dt>
-//          <max_gjk_iterations> <max_epa_iterations> <epa_tolerance> <fp64_islands>
-//          <floor_pos x y z> <floor_scale x y z> <floor_mass> <floor_moi>      (one line)
-//   worlds <nworlds>
-//   world <nbodies> <friction|-> <restitution|->      (then nbodies body lines)
-//   body <box|sphere> <px py pz> <vx vy vz> <wx wy wz> <ax ay az> <sx sy sz> <mass> <moi>
-// Boxes precede spheres within a world; for a sphere sx is the radius.
+// Scene files are read by nans_scene_io.hpp (format described there).
 //
 // Trajectory file (binary, little endian):
 //   char[8] "NANSTRJ1"; int32 nbodies, nframes, steps_per_frame;
@@ -25,7 +16,26 @@
 //
 // Usage: nans_headless --lib libnans_gpu.so --scene scene.txt --steps N
 //                      [--every K] [--mode plain|batched] [--out traj.bin] [--parse-only]
+//
+// Platform-loop mode, the reference's frame loop (code/sdl_nans.cpp:541-555,
+// 913-1000) around a SimUpdateAndRender library (libnans_sim.so, interface in
+// include/nans_sim.h):
+//        nans_headless --sim libnans_sim.so --frames N [--dt D] [--reload-every K]
+//                      [--input script.txt] [--scene scene.txt] [--lib libnans_gpu.so]
+//                      [--every K] [--out traj.bin]
+// It maps the reference's 64 MiB PermanentStorage and 256 MiB TransientStorage,
+// calls SimUpdateAndRender once per frame with a fixed dt (default 1/60), and
+// unloads and reloads the library every K frames as the reference does when the
+// library's file changes (code/sdl_nans.cpp:919-927).  --scene and --lib are
+// passed on as NANS_SIM_SCENE and NANS_GPU_LIB.  The input script holds lines
+//   <frame> <button> <0|1>          (button held from that frame on, or released)
+//   <frame> mouse <xrel> <yrel>     (relative mouse motion in that frame)
+// with buttons forward back left right shoot debug_up debug_down debug_left
+// debug_right debug_forward debug_back debug_reset debug_continue.  Trajectory
+// records are written after every K-th frame and after the last one; their step
+// field is the number of frames run.
 #include <dlfcn.h>
+#include <sys/mman.h>
 
 #include <chrono>
 #include <cmath>
@@ -36,6 +46,11 @@
 #include <vector>
 
 #include "../../include/nans_gpu.h"
+#include "../../include/nans_sim.h"
+#include "nans_scene_io.hpp"
+
+using nans_io::planes;
+using nans_io::scene;
 
 struct nans_api {
     void *lib;
@@ -75,113 +90,6 @@ static bool load_api(nans_api *a, const char *path)
     return ok;
 }
 
-struct scene {
-    nans_params p;
-    std::vector<int32_t> offsets;
-    std::vector<float> fric, rest;     // per world (NaN: the params' value)
-    std::vector<int32_t> kind;
-    std::vector<float> pos, vel, angvel, angles, scale, mass, moi;  // [N][3] / [N] as read
-};
-
-static bool next_line(FILE *f, char *buf, int n)
-{
-    while (fgets(buf, n, f)) {
-        char *s = buf;
-        while (*s == ' ' || *s == '\t') ++s;
-        if (*s == '#' || *s == '\n' || *s == '\r' || *s == 0) continue;
-        return true;
-    }
-    return false;
-}
-
-static bool parse_coef(const char *tok, float *out)
-{
-    if (strcmp(tok, "-") == 0) {
-        *out = NAN;
-        return true;
-    }
-    char *end = nullptr;
-    *out = strtof(tok, &end);
-    return end && *end == 0;
-}
-
-// Reads a scene; p starts from the library defaults (or zeros with --parse-only).
-static bool read_scene(const char *path, scene *sc)
-{
-    FILE *f = fopen(path, "r");
-    if (!f) {
-        fprintf(stderr, "nans_headless: cannot open %s\n", path);
-        return false;
-    }
-    char line[1024];
-    bool ok = next_line(f, line, sizeof(line)) && strncmp(line, "nans-scene 1", 12) == 0;
-    if (!ok) fprintf(stderr, "nans_headless: %s is not a nans-scene 1 file\n", path);
-    nans_params &p = sc->p;
-    if (ok) {
-        ok = next_line(f, line, sizeof(line)) &&
-             sscanf(line, "params %d %f %f %f %f %f %f %d %d %f %d %f %f %f %f %f %f %f %f", &p.solver_iterations,
-                    &p.beta, &p.restitution, &p.friction, &p.damping, &p.gravity, &p.dt, &p.max_gjk_iterations,
-                    &p.max_epa_iterations, &p.epa_tolerance, &p.fp64_islands, &p.floor_pos[0], &p.floor_pos[1],
-                    &p.floor_pos[2], &p.floor_scale[0], &p.floor_scale[1], &p.floor_scale[2], &p.floor_mass,
-                    &p.floor_moi) == 19;
-        if (!ok) fprintf(stderr, "nans_headless: bad params line\n");
-    }
-    int nworlds = 0;
-    if (ok) {
-        ok = next_line(f, line, sizeof(line)) && sscanf(line, "worlds %d", &nworlds) == 1 && nworlds >= 1;
-        if (!ok) fprintf(stderr, "nans_headless: bad worlds line\n");
-    }
-    sc->offsets.assign(1, 0);
-    for (int w = 0; ok && w < nworlds; ++w) {
-        int nb = 0;
-        char fr[64], rs[64];
-        ok = next_line(f, line, sizeof(line)) && sscanf(line, "world %d %63s %63s", &nb, fr, rs) == 3 && nb >= 0;
-        float cf = 0.0f, cr = 0.0f;
-        ok = ok && parse_coef(fr, &cf) && parse_coef(rs, &cr);
-        if (!ok) {
-            fprintf(stderr, "nans_headless: bad world line %d\n", w);
-            break;
-        }
-        sc->fric.push_back(cf);
-        sc->rest.push_back(cr);
-        for (int i = 0; ok && i < nb; ++i) {
-            char kind[16];
-            float v[17];
-            ok = next_line(f, line, sizeof(line)) &&
-                 sscanf(line, "body %15s %f %f %f %f %f %f %f %f %f %f %f %f %f %f %f %f %f", kind, &v[0], &v[1],
-                        &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10], &v[11], &v[12], &v[13],
-                        &v[14], &v[15], &v[16]) == 18 &&
-                 (strcmp(kind, "box") == 0 || strcmp(kind, "sphere") == 0);
-            if (!ok) {
-                fprintf(stderr, "nans_headless: bad body line (world %d, body %d)\n", w, i);
-                break;
-            }
-            sc->kind.push_back(strcmp(kind, "box") == 0 ? NANS_SHAPE_BOX : NANS_SHAPE_SPHERE);
-            for (int c = 0; c < 3; ++c) {
-                sc->pos.push_back(v[c]);
-                sc->vel.push_back(v[3 + c]);
-                sc->angvel.push_back(v[6 + c]);
-                sc->angles.push_back(v[9 + c]);
-                sc->scale.push_back(v[12 + c]);
-            }
-            sc->mass.push_back(v[15]);
-            sc->moi.push_back(v[16]);
-        }
-        sc->offsets.push_back((int32_t)sc->kind.size());
-    }
-    fclose(f);
-    return ok;
-}
-
-// [N][3] -> [3][N] planes
-static std::vector<float> planes(const std::vector<float> &a, size_t n)
-{
-    std::vector<float> out(3 * n);
-    for (size_t i = 0; i < n; ++i)
-        for (int c = 0; c < 3; ++c) out[c * n + i] = a[3 * i + c];
-    return out;
-}
-
 static bool write_frame(FILE *f, int32_t step, const std::vector<float> &pos, const std::vector<float> &ang, size_t n)
 {
     std::vector<float> buf(6 * n);
@@ -193,10 +101,141 @@ static bool write_frame(FILE *f, int32_t step, const std::vector<float> &pos, co
     return fwrite(&step, 4, 1, f) == 1 && fwrite(buf.data(), sizeof(float), buf.size(), f) == buf.size();
 }
 
+
+struct input_event {
+    int frame, button;  // button -1: mouse
+    int a, b;
+};
+
+static bool read_input(const char *path, std::vector<input_event> *ev)
+{
+    static const char *names[NANS_SIM_BUTTON_COUNT] = {
+        "forward",    "back",        "left",          "right",      "shoot",       "debug_up",      "debug_down",
+        "debug_left", "debug_right", "debug_forward", "debug_back", "debug_reset", "debug_continue"};
+    FILE *f = fopen(path, "r");
+    if (!f) {
+        fprintf(stderr, "nans_headless: cannot open %s\n", path);
+        return false;
+    }
+    char line[256], name[64];
+    bool ok = true;
+    while (ok && nans_io::next_line(f, line, sizeof(line))) {
+        input_event e{0, -1, 0, 0};
+        ok = sscanf(line, "%d %63s %d %d", &e.frame, name, &e.a, &e.b) >= 3 && e.frame >= 0;
+        if (ok && strcmp(name, "mouse") != 0) {
+            e.button = -2;
+            for (int k = 0; k < NANS_SIM_BUTTON_COUNT; ++k)
+                if (strcmp(name, names[k]) == 0) e.button = k;
+            ok = e.button >= 0;
+        }
+        if (!ok) fprintf(stderr, "nans_headless: bad input line: %s", line);
+        else ev->push_back(e);
+    }
+    fclose(f);
+    return ok;
+}
+
+struct sim_code {
+    void *lib;
+    void (*update)(nans_sim_memory *, nans_sim_input *, void *, float);
+    void (*release)(nans_sim_memory *);
+};
+
+// SDLLoadSimCode (code/sdl_nans.cpp:395-418)
+static bool load_sim(sim_code *c, const char *path)
+{
+    c->lib = dlopen(path, RTLD_NOW | RTLD_GLOBAL);
+    if (!c->lib) {
+        fprintf(stderr, "nans_headless: cannot load %s: %s\n", path, dlerror());
+        return false;
+    }
+    *(void **)&c->update = dlsym(c->lib, "SimUpdateAndRender");
+    *(void **)&c->release = dlsym(c->lib, "nans_sim_release");
+    if (!c->update) fprintf(stderr, "nans_headless: %s lacks SimUpdateAndRender\n", path);
+    return c->update != nullptr;
+}
+
+static int run_sim(const char *simpath, const char *libpath, const char *scenepath, const char *inputpath,
+                   const char *outpath, int frames, float dt, int reload_every, int every)
+{
+    if (libpath) setenv("NANS_GPU_LIB", libpath, 1);
+    if (scenepath) setenv("NANS_SIM_SCENE", scenepath, 1);
+    std::vector<input_event> events;
+    if (inputpath && !read_input(inputpath, &events)) return 1;
+    nans_sim_memory mem = {};
+    mem.PermanentStorageSize = 64ull << 20;
+    mem.TransientStorageSize = 256ull << 20;
+    const size_t total = mem.PermanentStorageSize + mem.TransientStorageSize;
+    void *base = mmap(nullptr, total, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+    if (base == MAP_FAILED) {
+        fprintf(stderr, "nans_headless: cannot map %zu bytes\n", total);
+        return 1;
+    }
+    mem.PermanentStorage = base;
+    mem.TransientStorage = (char *)base + mem.PermanentStorageSize;
+    const nans_sim_state *st = (const nans_sim_state *)mem.PermanentStorage;
+    sim_code code;
+    if (!load_sim(&code, simpath)) return 1;
+    nans_sim_input in = {};
+    FILE *out = nullptr;
+    const int per = every > 0 ? every : (frames > 0 ? frames : 1);
+    int reloads = 0, rc = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int frame = 0; frame < frames && rc == 0; ++frame) {
+        if (reload_every > 0 && frame > 0 && frame % reload_every == 0) {
+            dlclose(code.lib);  // SDLUnloadSimCode (code/sdl_nans.cpp:420-431)
+            if (!load_sim(&code, simpath)) return 1;
+            ++reloads;
+        }
+        in.Sensitivity = 0.5f;  // code/sdl_nans.cpp:939
+        in.XRel = in.YRel = 0;
+        for (const input_event &e : events)
+            if (e.frame == frame) {
+                if (e.button < 0) in.XRel = e.a, in.YRel = e.b;
+                else in.Buttons[e.button].EndedDown = e.a != 0;
+            }
+        code.update(&mem, &in, nullptr, dt);
+        if (st->magic != NANS_SIM_MAGIC || st->status != NANS_OK) {
+            fprintf(stderr, "nans_headless: frame %d: %s\n", frame, st->magic == NANS_SIM_MAGIC ? st->error :
+                                                                     "the simulation did not initialise its state");
+            rc = 1;
+            break;
+        }
+        const int done = frame + 1;
+        if (outpath && (done % per == 0 || done == frames)) {
+            const int n = st->nboxes + st->nspheres;
+            if (!out) {
+                out = fopen(outpath, "wb");
+                if (!out) {
+                    fprintf(stderr, "nans_headless: cannot write %s\n", outpath);
+                    rc = 1;
+                    break;
+                }
+                const int32_t hdr[3] = {n, (frames + per - 1) / per, per};
+                fwrite("NANSTRJ1", 1, 8, out);
+                fwrite(hdr, 4, 3, out);
+            }
+            std::vector<float> P(st->pos, st->pos + 3 * n), A(st->angles, st->angles + 3 * n);
+            if (!write_frame(out, done, P, A, (size_t)n)) rc = 1;
+        }
+    }
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rc == 0)
+        printf("sim: %d frames, %d bodies, %d reloads, %d uploads in %.3f s\n", st->frames,
+               st->nboxes + st->nspheres, reloads, st->uploads, wall);
+    if (out) fclose(out);
+    if (code.release) code.release(&mem);
+    dlclose(code.lib);
+    munmap(base, total);
+    return rc;
+}
+
 int main(int argc, char **argv)
 {
     const char *libpath = nullptr, *scenepath = nullptr, *outpath = nullptr, *mode = "plain";
-    int steps = 0, every = 0;
+    const char *simpath = nullptr, *inputpath = nullptr;
+    int steps = 0, every = 0, frames = 0, reload_every = 0;
+    float dt = 1.0f / 60.0f;
     bool parse_only = false;
     for (int i = 1; i < argc; ++i) {
         auto arg = [&](const char *name) { return strcmp(argv[i], name) == 0 && i + 1 < argc; };
@@ -206,21 +245,35 @@ int main(int argc, char **argv)
         else if (arg("--mode")) mode = argv[++i];
         else if (arg("--steps")) steps = atoi(argv[++i]);
         else if (arg("--every")) every = atoi(argv[++i]);
+        else if (arg("--sim")) simpath = argv[++i];
+        else if (arg("--input")) inputpath = argv[++i];
+        else if (arg("--frames")) frames = atoi(argv[++i]);
+        else if (arg("--reload-every")) reload_every = atoi(argv[++i]);
+        else if (arg("--dt")) dt = strtof(argv[++i], nullptr);
         else if (strcmp(argv[i], "--parse-only") == 0) parse_only = true;
         else {
             fprintf(stderr, "nans_headless: unknown argument %s\n", argv[i]);
             return 2;
         }
     }
+    if (simpath) {
+        if (frames < 0 || every < 0 || reload_every < 0) {
+            fprintf(stderr, "nans_headless: --frames, --every and --reload-every must be >= 0\n");
+            return 2;
+        }
+        return run_sim(simpath, libpath, scenepath, inputpath, outpath, frames, dt, reload_every, every);
+    }
     if (!scenepath || (!parse_only && !libpath) || steps < 0 || every < 0 ||
         (strcmp(mode, "plain") != 0 && strcmp(mode, "batched") != 0)) {
         fprintf(stderr, "usage: nans_headless --lib libnans_gpu.so --scene scene.txt --steps N [--every K] "
-                        "[--mode plain|batched] [--out traj.bin] [--parse-only]\n");
+                        "[--mode plain|batched] [--out traj.bin] [--parse-only]\n"
+                        "       nans_headless --sim libnans_sim.so --frames N [--dt D] [--reload-every K] "
+                        "[--input script.txt] [--scene scene.txt] [--lib libnans_gpu.so] [--every K] [--out traj.bin]\n");
         return 2;
     }
     scene sc;
     memset(&sc.p, 0, sizeof(sc.p));
-    if (!read_scene(scenepath, &sc)) return 1;
+    if (!nans_io::read_scene(scenepath, &sc, "nans_headless")) return 1;
     const size_t n = sc.kind.size();
     const int nworlds = (int)sc.offsets.size() - 1;
     if (parse_only) {

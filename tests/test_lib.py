@@ -1,5 +1,5 @@
-"""CPU checks of the C ABI: libnans_gpu.so loads, exports every function that
-include/nans_gpu.h declares, reports the reference's constants as defaults and
+"""CPU checks of the C ABI: libnans_gpu.so and libnans_sim.so load and export
+every function that include/nans_gpu.h and include/nans_sim.h declare, reports the reference's constants as defaults and
 rejects bad arguments with an error code and message (no GPU work)."""
 import ctypes
 import os
@@ -12,20 +12,32 @@ from nans_projekat_amd import runtime
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    names = set()
-    for fn in os.listdir(os.path.join(ROOT, "include")):
-        if fn.endswith(".h"):
-            text = open(os.path.join(ROOT, "include", fn)).read()
-            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-            names |= set(re.findall(r"\b(nans_\w+)\s*\(", text))
-    return sorted(names)
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b((?:nans_|Sim)\w+)\s*\(", text)))
+
+
+def test_every_header_is_checked():
+    assert sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")) == \
+        ["nans_gpu.h", "nans_sim.h"]
 
 
 def test_library_exports_every_declared_function():
     lib = runtime.load()
-    names = declared_functions()
+    names = declared_functions("nans_gpu.h")
     assert len(names) >= 7
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_plugin_exports_every_declared_function():
+    from nans_projekat_amd import build
+    if not os.path.exists(build.SIM):
+        build.build_driver()
+    lib = ctypes.CDLL(build.SIM)
+    names = declared_functions("nans_sim.h")
+    assert names == ["SimUpdateAndRender", "nans_sim_release"]
     for name in names:
         assert hasattr(lib, name), name
 
