@@ -209,6 +209,18 @@ int nans_batched_timing(nans_ctx *ctx, nans_timing *timing);
  * uploaded batch in place, keeping buffers and the captured graph. */
 int nans_set_state(nans_ctx *ctx, const nans_bodies_soa *bodies);
 
+/* Broadphase of the plain step.  The reference tests every pair of a world
+ * (code/nans.cpp:1355-1535); the plain step keeps the pairs whose bounding
+ * spheres can touch, found either by testing every canonical pair or with a
+ * uniform grid.  Both give the same candidates in the same canonical order, so
+ * results are identical; AUTO (the default) takes the grid for worlds of 1024
+ * bodies or more.  The batched step always tests every pair of its (small)
+ * worlds. */
+#define NANS_BROADPHASE_AUTO 0
+#define NANS_BROADPHASE_ALL_PAIRS 1
+#define NANS_BROADPHASE_GRID 2
+int nans_set_broadphase(nans_ctx *ctx, int32_t mode);
+
 /* Autotuner of the batched step (no counterpart in the reference, which steps one
  * scene on one thread).  Each workgroup of the batched step runs one world: its
  * size sets how many canonical pairs are culled and GJK-tested at once, how many

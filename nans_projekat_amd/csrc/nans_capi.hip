@@ -215,6 +215,15 @@ int nans_set_stream(nans_ctx *ctx, void *hip_stream)
     return NANS_OK;
 }
 
+int nans_set_broadphase(nans_ctx *ctx, int32_t mode)
+{
+    NANS_REQUIRE(ctx, NANS_ERR_INVALID, "ctx is NULL");
+    NANS_REQUIRE(mode == NANS_BROADPHASE_AUTO || mode == NANS_BROADPHASE_ALL_PAIRS || mode == NANS_BROADPHASE_GRID,
+                 NANS_ERR_INVALID, "broadphase mode %d is not one of NANS_BROADPHASE_*", (int)mode);
+    ctx->broadphase = mode;
+    return NANS_OK;
+}
+
 int nans_narrowphase(nans_ctx *ctx, int64_t n, const nans_shapes_soa *a, const nans_shapes_soa *b,
                      const nans_narrow_out *out, int64_t *n_overflow)
 {

@@ -61,4 +61,5 @@ struct nans_ctx {
     int64_t np_cap;                  // narrowphase entry point: queue capacity (pairs)
     int *np_qidx;                    // queued GJK hits
     void *np_qsim;                   // their simplices (SimplexRec)
+    int broadphase;                  // NANS_BROADPHASE_* of the plain step
 };

@@ -320,12 +320,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int *counts, int nb
         running += tot;
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && total) {
         *total = min(running, cap);
         atomicAdd(&cnt[CNT_CANDIDATES], (unsigned long long)running);
         if (running > cap) atomicAdd(&cnt[CNT_CAND_OVERFLOW], 1ull);
     }
 }
+
+}  // namespace nans
+#include "nans_broadphase.hpp"
+namespace nans {
 
 constexpr int NARROW_RANGE = 512;  // candidates per workgroup of the plain narrowphase
 
@@ -612,7 +616,10 @@ int worlds_release(nans_ctx *ctx)
                     (void *)W->order, (void *)W->wfric, (void *)W->wrest, (void *)W->floor, (void *)W->cnt,
                     (void *)W->cand, (void *)W->rec, (void *)W->flag, (void *)W->con, (void *)W->done,
                     (void *)W->qidx, (void *)W->qsim,
-                    (void *)W->blk_counts, (void *)W->blk_offsets, (void *)W->ncand, (void *)W->queue})
+                    (void *)W->blk_counts, (void *)W->blk_offsets, (void *)W->ncand, (void *)W->queue,
+                    (void *)W->bp_cell, (void *)W->bp_hslot, (void *)W->bp_sorted, (void *)W->bp_wild,
+                    (void *)W->bp_bcnt, (void *)W->bp_boff, (void *)W->bp_counts, (void *)W->bp_offsets,
+                    W->bp_grid})
         wfree(p);
     for (hipEvent_t ev : {W->ev_in, W->ev_out, W->ev_t0, W->ev_t1})
         if (ev) hipEventDestroy(ev);
@@ -688,6 +695,58 @@ int check_overflow(nans_ctx *ctx)
     NANS_REQUIRE(c[CNT_CAND_OVERFLOW] == 0 && c[CNT_CON_OVERFLOW] == 0, NANS_ERR_CAPACITY,
                  "pair candidates or contacts exceeded the scratch capacity");
     return NANS_OK;
+}
+
+}  // namespace nans
+
+namespace nans {
+
+// The grid broadphase replaces the all-pairs cull for worlds of at least this
+// many bodies (both give the same candidates).
+constexpr int GRID_MIN_BODIES = 1024;
+
+static bool use_grid(const nans_ctx *ctx, int n)
+{
+    if (ctx->broadphase == NANS_BROADPHASE_GRID) return true;
+    if (ctx->broadphase == NANS_BROADPHASE_ALL_PAIRS) return false;
+    return n >= GRID_MIN_BODIES;
+}
+
+// Candidates of one world from the uniform grid (nans_broadphase.hpp), in
+// canonical order, into W->cand / W->ncand.
+static void grid_broadphase(nans_worlds *W, const Planes &B, const float *brad, int nb, int ns, int64_t npairs,
+                            hipStream_t s)
+{
+    const int n = nb + ns, blocks = (n + 255) / 256;
+    BpGrid *g = (BpGrid *)W->bp_grid;
+    const unsigned mask = (unsigned)W->bp_buckets - 1;
+    k_bp_bounds<<<1, BP_THREADS, 0, s>>>(B, brad, n, W->bp_bcnt, W->bp_buckets, g, W->bp_wild, W->cnt,
+                                          (long long)npairs);
+    k_bp_cells<<<blocks, 256, 0, s>>>(B, brad, n, g, mask, W->bp_cell, W->nmax, W->bp_hslot, W->bp_bcnt);
+    k_scan<<<1, SCAN_THREADS, 0, s>>>(W->bp_bcnt, W->bp_buckets, W->bp_boff, nullptr, 0, nullptr);
+    k_bp_scatter<<<blocks, 256, 0, s>>>(n, W->bp_hslot, W->nmax, W->bp_boff, W->bp_sorted);
+    BpRows r;
+    r.B = B;
+    r.brad = brad;
+    r.nb = nb;
+    r.ns = ns;
+    r.fl = W->floor;
+    r.g = g;
+    r.mask = mask;
+    r.cell = W->bp_cell;
+    r.stride = W->nmax;
+    r.hslot = W->bp_hslot;
+    r.bucket_cnt = W->bp_bcnt;
+    r.bucket_off = W->bp_boff;
+    r.sorted = W->bp_sorted;
+    r.wild = W->bp_wild;
+    r.counts = W->bp_counts;
+    r.offsets = W->bp_offsets;
+    r.cand = W->cand;
+    r.cap = W->cand_cap;
+    k_bp_rows<false><<<blocks, 256, 0, s>>>(r);
+    k_scan<<<1, SCAN_THREADS, 0, s>>>(W->bp_counts, 3 * nb + 2 * ns, W->bp_offsets, W->ncand, W->cand_cap, W->cnt);
+    k_bp_rows<true><<<blocks, 256, 0, s>>>(r);
 }
 
 }  // namespace nans
@@ -794,6 +853,17 @@ int nans_upload_worlds(nans_ctx *ctx, int32_t nworlds, const int32_t *world_offs
     NANS_TRY(dmalloc(&W->blk_counts, (size_t)W->max_blk));
     NANS_TRY(dmalloc(&W->blk_offsets, (size_t)W->max_blk));
     NANS_TRY(dmalloc(&W->ncand, 1));
+    W->bp_buckets = 64;
+    while (W->bp_buckets < 2 * nmax) W->bp_buckets *= 2;
+    NANS_TRY(dmalloc(&W->bp_cell, (size_t)3 * nmax));
+    NANS_TRY(dmalloc(&W->bp_hslot, (size_t)2 * nmax));
+    NANS_TRY(dmalloc(&W->bp_sorted, (size_t)nmax));
+    NANS_TRY(dmalloc(&W->bp_wild, (size_t)nmax));
+    NANS_TRY(dmalloc(&W->bp_bcnt, (size_t)W->bp_buckets));
+    NANS_TRY(dmalloc(&W->bp_boff, (size_t)W->bp_buckets));
+    NANS_TRY(dmalloc(&W->bp_counts, (size_t)3 * nmax));
+    NANS_TRY(dmalloc(&W->bp_offsets, (size_t)3 * nmax));
+    NANS_TRY(dmalloc((BpGrid **)&W->bp_grid, 1));
     NANS_HIP(hipFuncSetAttribute((const void *)k_plain_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)(sizeof(int) * PLAIN_MAX_BODIES)));
     k_zero_u64<<<1, 64>>>(W->cnt, CNT_N);
@@ -930,11 +1000,16 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
             const int nblk = (int)((npairs + CULL_CHUNK - 1) / CULL_CHUNK);
             k_plain_forces<<<(n + 255) / 256, 256, 0, s>>>(B, verts, brad, n, nb, k);
             if (nblk > 0) {
-                k_plain_cull<false><<<nblk, 256, 0, s>>>(B, brad, nb, ns, npairs, W->floor, W->blk_counts, nullptr,
-                                                         nullptr, 0, W->cnt);
-                k_scan<<<1, SCAN_THREADS, 0, s>>>(W->blk_counts, nblk, W->blk_offsets, W->ncand, W->cand_cap, W->cnt);
-                k_plain_cull<true><<<nblk, 256, 0, s>>>(B, brad, nb, ns, npairs, W->floor, nullptr, W->blk_offsets,
-                                                        W->cand, W->cand_cap, W->cnt);
+                if (use_grid(ctx, n)) {
+                    grid_broadphase(W, B, brad, nb, ns, npairs, s);
+                } else {
+                    k_plain_cull<false><<<nblk, 256, 0, s>>>(B, brad, nb, ns, npairs, W->floor, W->blk_counts,
+                                                             nullptr, nullptr, 0, W->cnt);
+                    k_scan<<<1, SCAN_THREADS, 0, s>>>(W->blk_counts, nblk, W->blk_offsets, W->ncand, W->cand_cap,
+                                                      W->cnt);
+                    k_plain_cull<true><<<nblk, 256, 0, s>>>(B, brad, nb, ns, npairs, W->floor, nullptr,
+                                                            W->blk_offsets, W->cand, W->cand_cap, W->cnt);
+                }
                 int grid = std::max(1, std::min((W->cand_cap + NARROW_RANGE - 1) / NARROW_RANGE, 2048));
                 k_plain_narrow<<<grid, 256, 0, s>>>(B, verts, nb, ns, W->floor, W->cand, W->ncand, k.np, W->rec,
                                                     W->flag, W->qidx, W->qsim, W->cnt);

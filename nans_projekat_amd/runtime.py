@@ -296,6 +296,13 @@ class Context:
     def set_batched_block(self, threads_per_world: int) -> None:
         check(self.lib.nans_set_batched_block(self.h, int(threads_per_world)))
 
+    BROADPHASE = {"auto": 0, "all_pairs": 1, "grid": 2}
+
+    def set_broadphase(self, mode: str) -> None:
+        """Broadphase of the plain step: "auto", "all_pairs" or "grid" (same
+        candidates and results either way)."""
+        check(self.lib.nans_set_broadphase(self.h, self.BROADPHASE[mode]))
+
     def batched_timing(self) -> dict:
         t = Timing()
         check(self.lib.nans_batched_timing(self.h, ctypes.byref(t)))
