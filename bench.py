@@ -14,7 +14,8 @@ Also reported on the same line:
                   path) on a size-stratified subset of the same worlds, and the
                   batched/plain ratio;
   narrowphase     config 2 (1M box/sphere pairs, GJK + EPA) pairs/s;
-  large_world     config 5 (one world of 8192 boxes) through the plain step;
+  large_world     config 5 (one world of 8192 boxes) through the plain step with the
+                  grid broadphase, and with the all-pairs cull for comparison;
   differentiable  config 4 (1024 x 16 cubes, 240 steps): checkpointed rollout
                   plus the hand-written adjoint, body-steps/s;
   roofline        the batched step kernel: algorithmic FP32 flops per launch
@@ -258,21 +259,28 @@ def main():
                               "kernel_ms": round(e0.elapsed_time(e1) / args.np_reps, 4),
                               "workload": "config2: 1M pairs (50% box-box, 30% box-sphere, 20% sphere-sphere)"}
         nctx.close()
-        # large single world (config 5) through the plain step
+        # large single world (config 5) through the plain step: the grid broadphase
+        # (the default for worlds this size) and, for comparison, the all-pairs cull
         big = S.config5()
         lctx = runtime.Context(big.params)
-        lctx.upload(big)
-        torch.cuda.synchronize()
-        l0 = time.perf_counter()
-        lctx.step_plain(args.large_steps)
-        torch.cuda.synchronize()
-        lwall = time.perf_counter() - l0
+        lwall = {}
+        for mode in ("all_pairs", "grid"):
+            lctx.upload(big)
+            lctx.set_broadphase(mode)
+            torch.cuda.synchronize()
+            l0 = time.perf_counter()
+            lctx.step_plain(args.large_steps)
+            torch.cuda.synchronize()
+            lwall[mode] = time.perf_counter() - l0
         lst = lctx.stats()
-        out["large_world"] = {"value": round(big.nbodies * args.large_steps / lwall, 1), "unit": "body-steps/s",
-                              "ms_per_step": round(1e3 * lwall / args.large_steps, 4),
-                              "workload": "config5: 8192 boxes, dt 1/240, 20 iterations, %d steps, plain step"
-                                          % args.large_steps,
-                              "pair_tests_per_s": round(lst["pair_tests"] / lwall, 1),
+        out["large_world"] = {"value": round(big.nbodies * args.large_steps / lwall["grid"], 1),
+                              "unit": "body-steps/s",
+                              "ms_per_step": round(1e3 * lwall["grid"] / args.large_steps, 4),
+                              "workload": "config5: 8192 boxes, dt 1/240, 20 iterations, %d steps, plain step, "
+                                          "grid broadphase" % args.large_steps,
+                              "all_pairs_ms_per_step": round(1e3 * lwall["all_pairs"] / args.large_steps, 4),
+                              "canonical_pairs_per_s": round(lst["pair_tests"] / lwall["grid"], 1),
+                              "candidates": lst["candidates"],
                               "contacts": lst["contacts"]}
         lctx.close()
         # differentiable rollout (config 4): checkpointed forward + hand-written adjoint

@@ -119,16 +119,16 @@ struct nans_worlds {
     int *blk_offsets = nullptr;
     int *ncand = nullptr;
     int max_blk = 0;
-    // grid broadphase of the plain step (sized for the largest world)
+    // hierarchical-grid broadphase of the plain step (sized for the largest world)
     int bp_buckets = 0;          // power of two >= 2 * nmax
     long long *bp_cell = nullptr;  // [3][nmax] cell coordinates
-    int *bp_hslot = nullptr;     // [2][nmax] bucket, slot in bucket
+    int *bp_hslot = nullptr;     // [3][nmax] bucket, slot in bucket, level
     int *bp_sorted = nullptr;    // [nmax] bodies grouped by bucket
-    int *bp_wild = nullptr;      // [nmax]
     int *bp_bcnt = nullptr;      // [bp_buckets]
     int *bp_boff = nullptr;      // [bp_buckets]
     int *bp_counts = nullptr;    // [3 nmax] row counts
     int *bp_offsets = nullptr;   // [3 nmax]
+    int *bp_tmp = nullptr;       // [cand_cap] unsorted rows
     void *bp_grid = nullptr;     // nans::BpGrid
     // batched scratch
     int b_grid = 0;

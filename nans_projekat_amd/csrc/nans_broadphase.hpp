@@ -1,49 +1,81 @@
-// nans_broadphase.hpp -- uniform-grid broadphase of the plain step.
+// nans_broadphase.hpp -- hierarchical-grid broadphase of the plain step.
 //
 // The reference tests every pair of a world (code/nans.cpp:1355-1535).  The
 // plain step's all-pairs cull (k_plain_cull) keeps the canonical pairs for which
 // cull_keep holds and hands them to the narrowphase in canonical order.  The grid
-// below finds exactly the same pairs with O(n) expected work and writes them in
-// the same order, so the narrowphase, the solver and every result are unchanged.
+// below finds exactly the same pairs without testing every one and writes them
+// in the same order, so the narrowphase, the solver and every result are
+// unchanged.
 //
-// Why no pair is lost.  A body is tame when |x|+|y|+|z| of its centre and the
-// magnitude of its bounding radius are both <= 1e18; the others (NaN, inf, huge)
-// are wild and are paired with every body by an exhaustive scan, so only pairs of
-// two tame bodies rely on the grid.  For two tame bodies no term of cull_keep
-// overflows, and cull_keep(a, b) implies, per axis,
-//     |x_a - x_b| <= |lim| (1 + 4u),   |lim| <= (2R * 1.0001 + 1e-4 (1 + 2M + 2R)) (1 + 8u)
-// with u = 2^-24, R the largest tame |radius| and M the largest tame
-// |x|+|y|+|z|.  The cell size h is that bound's binary64 value times 1.01, so
-// |x_a - x_b| / h < 0.991.  Cell coordinates are floor(x * (1/h)) in binary64,
-// clamped to +-2^40: below 2^41 cells the product is within 2^-11 of x / h, so
-// the two products of a kept pair differ by less than 0.991 + 2^-10 < 1 and
-// their floors by at most one, which clamping (1-Lipschitz) keeps; a pair with a
-// centre beyond 2^41 cells has both centres beyond 2^40 on the same side, and
-// both clamp to the same cell.  Hence the 27 cells around a body's cell hold
-// every tame partner that cull_keep accepts.  Cells are hashed into buckets; a bucket entry counts only
-// when its own cell equals the searched cell, so each pair is found once.
+// Reach.  cull_keep's limit for a pair grows with the pair's radii and with the
+// magnitude of its centres.  Each body gets a reach
+//     rho = (|r| * 1.0002 + 1e-4 * (0.5 + |x|+|y|+|z|)) * (1 + 4e-6)      (binary64)
+// such that for two tame bodies (below) cull_keep(a, b) implies, per axis,
+// |x_a - x_b| <= rho_a + rho_b: the limit is at most
+// (|r_a|+|r_b|) * 1.0001 + 1e-4 (1 + mag_a + mag_b + |r_a| + |r_b|) up to binary32
+// rounding of its evaluation and of the magnitude sums (a relative 2e-6 at most),
+// and the distance test loses at most a further 4 * 2^-24; the factor 1 + 4e-6
+// covers both.
 //
-// Output.  Each pair is written by the body that owns its canonical row: a box
-// writes its CC row (boxes j > i), its CF flag and its CS row (every sphere); a
-// sphere writes its SF flag and its SS row (spheres t > s).  One exclusive scan
-// over the per-body counts laid out as [CC rows | CF | SF | CS rows | SS rows]
-// gives every write offset in canonical order; each row is then sorted in place.
+// Levels.  Level k has cells of size h_k = 2.02 * rho0 * 2^k, where rho0 is the
+// reach of the largest body at magnitude 128; a body lives in the lowest level
+// with rho <= rho0 * 2^k.  For a kept pair with levels k_a <= k_b,
+// |x_a - x_b| <= rho0 (2^k_a + 2^k_b) <= h_k_b / 1.01.  Cell coordinates are
+// floor(x * (1 / h_k)) in binary64, clamped to +-2^40: below 2^41 cells the
+// product is within 2^-11 of x / h_k, so the two coordinates of a kept pair
+// differ by less than 1/1.01 + 2^-10 < 1 and their floors by at most one, which
+// clamping (1-Lipschitz) keeps; a pair with a centre beyond 2^41 cells has both
+// centres beyond 2^40 on the same side, and both clamp to the same cell.  So a
+// body that searches the 27 cells around its centre at its own level and at
+// every coarser level finds every partner of the same or a coarser level, and a
+// pair is found once: by the finer body, or at equal levels by the body with
+// the smaller index.  Cells of all levels share one hash table; a bucket entry
+// counts only when its level and cell equal the searched ones.
+//
+// Wild bodies.  A body is tame when |x|+|y|+|z| and |r| are both <= 1e15 (which
+// also bounds the level below 58 and keeps every term of cull_keep finite); the
+// others (NaN, inf, huge) are wild: each sweeps every body, taking every tame
+// partner and the wild partners of larger index.
+//
+// Output.  A found pair is credited to the canonical row of its owner (CC: the
+// smaller box, CS: the box, SS: the smaller sphere).  Rows are laid out as
+// [CC rows | CF flags | SF flags | CS rows | SS rows]; one exclusive scan of the
+// per-row counts gives every row's place in canonical order, the second pass
+// drops each pair into its row, and each row is finally sorted.
 #pragma once
 
 namespace nans {
 
 constexpr int BP_THREADS = 1024;
 constexpr double BP_CLAMP = 1099511627776.0;  // 2^40 cells per axis
+constexpr float BP_TAME = 1e15f;
+constexpr int BP_LEVELS = 64;
 
 struct BpGrid {
-    double inv;  // 1 / cell size
-    int nwild;
-    int pad;
+    double rho0;                // reach covered by level 0
+    double inv0;                // 1 / cell size of level 0; level k: inv0 / 2^k
+    unsigned long long levels;  // bit k: level k holds a body
 };
 
 __device__ inline float bp_mag(V3 p) { return fabsf(p.x) + fabsf(p.y) + fabsf(p.z); }
 
-__device__ inline bool bp_tame(V3 p, float r) { return bp_mag(p) <= 1e18f && fabsf(r) <= 1e18f; }
+__device__ inline bool bp_tame(V3 p, float r) { return bp_mag(p) <= BP_TAME && fabsf(r) <= BP_TAME; }
+
+__device__ inline double bp_reach(V3 p, float r)
+{
+    return ((double)fabsf(r) * 1.0002 + 1e-4 * (0.5 + (double)bp_mag(p))) * (1.0 + 4e-6);
+}
+
+__device__ inline int bp_level(double rho, double rho0)
+{
+    int k = 0;
+    double lim = rho0;
+    while (rho > lim && k < BP_LEVELS - 1) {
+        lim *= 2.0;
+        ++k;
+    }
+    return k;
+}
 
 __device__ inline long long bp_coord(float x, double inv)
 {
@@ -52,11 +84,12 @@ __device__ inline long long bp_coord(float x, double inv)
     return (long long)u;
 }
 
-__device__ inline unsigned bp_hash(long long x, long long y, long long z, unsigned mask)
+__device__ inline unsigned bp_hash(int level, long long x, long long y, long long z, unsigned mask)
 {
     unsigned long long h = (unsigned long long)x * 0x9E3779B97F4A7C15ull;
     h ^= (unsigned long long)y * 0xC2B2AE3D27D4EB4Full;
     h ^= (unsigned long long)z * 0x165667B19E3779F9ull;
+    h ^= (unsigned long long)(level + 1) * 0xD6E8FEB86659FD93ull;
     h ^= h >> 29;
     return (unsigned)h & mask;
 }
@@ -68,63 +101,64 @@ __device__ inline float bp_wave_max(float v)
     return v;
 }
 
-// One workgroup: clears the bucket counters, lists the wild bodies and sets the
-// cell size from the tame bodies' largest radius and centre magnitude.
+// One workgroup: clears the bucket and row counters and sets the level-0 reach
+// from the tame bodies' largest radius.
 __global__ __launch_bounds__(BP_THREADS) void k_bp_bounds(Planes B, const float *brad, int n, int *bucket_cnt,
-                                                          int nbuckets, BpGrid *g, int *wild,
+                                                          int nbuckets, int *counts, int ncounts, BpGrid *g,
                                                           unsigned long long *cnt, long long npairs)
 {
-    __shared__ float s_r[BP_THREADS / 64], s_m[BP_THREADS / 64];
-    __shared__ int s_nw;
-    if (threadIdx.x == 0) s_nw = 0;
+    __shared__ float s_r[BP_THREADS / 64];
     for (int k = threadIdx.x; k < nbuckets; k += BP_THREADS) bucket_cnt[k] = 0;
+    for (int k = threadIdx.x; k < ncounts; k += BP_THREADS) counts[k] = 0;
     __syncthreads();
-    float rmax = 0.0f, mmax = 0.0f;
+    float rmax = 0.0f;
     for (int i = threadIdx.x; i < n; i += BP_THREADS) {
         const V3 p = B.get3(F_PX, i);
         const float r = brad[i];
-        if (bp_tame(p, r)) {
-            rmax = fmaxf(rmax, fabsf(r));
-            mmax = fmaxf(mmax, bp_mag(p));
-        } else {
-            wild[atomicAdd(&s_nw, 1)] = i;
-        }
+        if (bp_tame(p, r)) rmax = fmaxf(rmax, fabsf(r));
     }
     rmax = bp_wave_max(rmax);
-    mmax = bp_wave_max(mmax);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) s_r[w] = rmax, s_m[w] = mmax;
+    if ((threadIdx.x & 63) == 0) s_r[threadIdx.x >> 6] = rmax;
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int k = 1; k < BP_THREADS / 64; ++k) rmax = fmaxf(rmax, s_r[k]), mmax = fmaxf(mmax, s_m[k]);
-        const double R = rmax, M = mmax;
-        const double lim = 2.0 * R * 1.0001 + 1e-4 * (1.0 + 2.0 * M + 2.0 * R);
-        g->inv = 1.0 / (lim * 1.01);
-        g->nwild = s_nw;
+        for (int k = 1; k < BP_THREADS / 64; ++k) rmax = fmaxf(rmax, s_r[k]);
+        const double rho0 = ((double)rmax * 1.0002 + 1e-4 * (0.5 + 128.0)) * (1.0 + 4e-6);
+        g->rho0 = rho0;
+        g->inv0 = 1.0 / (2.02 * rho0);
+        g->levels = 0ull;
         atomicAdd(&cnt[CNT_PAIR_TESTS], (unsigned long long)npairs);
     }
 }
 
-// Cell and bucket of every tame body (hslot = [bucket][slot in bucket], -1: wild).
-__global__ __launch_bounds__(256) void k_bp_cells(Planes B, const float *brad, int n, const BpGrid *g,
-                                                  unsigned mask, long long *cell, int stride, int *hslot,
-                                                  int *bucket_cnt)
+// Level, cell and bucket of every tame body: hslot = [bucket][slot][level]
+// planes (bucket -1: wild).
+__global__ __launch_bounds__(256) void k_bp_cells(Planes B, const float *brad, int n, BpGrid *g, unsigned mask,
+                                                  long long *cell, int stride, int *hslot, int *bucket_cnt)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const V3 p = B.get3(F_PX, i);
-    if (!bp_tame(p, brad[i])) {
-        hslot[i] = -1;
-        return;
+    unsigned long long bit = 0ull;
+    if (i < n) {
+        const V3 p = B.get3(F_PX, i);
+        const float r = brad[i];
+        if (!bp_tame(p, r)) {
+            hslot[i] = -1;
+        } else {
+            const int k = bp_level(bp_reach(p, r), g->rho0);
+            const double inv = ldexp(g->inv0, -k);
+            const long long cx = bp_coord(p.x, inv), cy = bp_coord(p.y, inv), cz = bp_coord(p.z, inv);
+            cell[i] = cx;
+            cell[stride + i] = cy;
+            cell[2 * stride + i] = cz;
+            const unsigned h = bp_hash(k, cx, cy, cz, mask);
+            hslot[i] = (int)h;
+            hslot[stride + i] = atomicAdd(&bucket_cnt[h], 1);
+            hslot[2 * stride + i] = k;
+            bit = 1ull << k;
+        }
     }
-    const double inv = g->inv;
-    const long long cx = bp_coord(p.x, inv), cy = bp_coord(p.y, inv), cz = bp_coord(p.z, inv);
-    cell[i] = cx;
-    cell[stride + i] = cy;
-    cell[2 * stride + i] = cz;
-    const unsigned h = bp_hash(cx, cy, cz, mask);
-    hslot[i] = (int)h;
-    hslot[stride + i] = atomicAdd(&bucket_cnt[h], 1);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) bit |= __shfl_xor(bit, off, 64);
+    if ((threadIdx.x & 63) == 0 && bit) atomicOr(&g->levels, bit);
 }
 
 __global__ __launch_bounds__(256) void k_bp_scatter(int n, const int *hslot, int stride, const int *bucket_off,
@@ -136,7 +170,7 @@ __global__ __launch_bounds__(256) void k_bp_scatter(int n, const int *hslot, int
     if (h >= 0) sorted[bucket_off[h] + hslot[stride + i]] = i;
 }
 
-struct BpRows {
+struct BpFind {
     Planes B;
     const float *brad;
     int nb, ns;
@@ -146,95 +180,212 @@ struct BpRows {
     const long long *cell;
     int stride;
     const int *hslot;
-    const int *bucket_cnt, *bucket_off, *sorted, *wild;
-    int *counts;           // [3 nb + 2 ns] counts (WRITE = false)
-    const int *offsets;    // their exclusive scan (WRITE = true)
+    const int *bucket_cnt, *bucket_off, *sorted;
+    int *counts;           // [3 nb + 2 ns] row counts
+    const int *offsets;    // their exclusive scan (WRITE)
+    int *tmp;              // rows before sorting (same offsets as cand)
     int *cand;
     int cap;
 };
 
-// Rows of every body: counts them (WRITE = false) or writes the canonical pair
-// indices and sorts each row (WRITE = true).
+// Canonical row (index into the counts layout) and index of the pair {i, j}, and
+// its owner a and partner b in cull_keep's argument order.
+struct BpPair {
+    int row, a, b;
+    int64_t p;
+};
+
+__device__ inline BpPair bp_pair(int i, int j, int nb, int ns)
+{
+    BpPair q;
+    const bool ci = i < nb, cj = j < nb;
+    const int64_t base_cs = tri(nb) + nb + ns;
+    if (ci && cj) {
+        q.a = min(i, j);
+        q.b = max(i, j);
+        q.row = q.a;
+        q.p = (int64_t)q.a * (2 * (int64_t)nb - q.a - 1) / 2 + (q.b - q.a - 1);
+    } else if (ci != cj) {
+        q.a = ci ? i : j;
+        q.b = ci ? j : i;
+        q.row = 2 * nb + ns + q.a;
+        q.p = base_cs + (int64_t)q.a * ns + (q.b - nb);
+    } else {
+        q.a = min(i, j);
+        q.b = max(i, j);
+        const int sa = q.a - nb, sb = q.b - nb;
+        q.row = 3 * nb + ns + sa;
+        q.p = base_cs + (int64_t)nb * ns + (int64_t)sa * (2 * (int64_t)ns - sa - 1) / 2 + (sb - sa - 1);
+    }
+    return q;
+}
+
+// Pairs of every body, one wave per body: counts them into their rows
+// (WRITE = false) or drops their canonical indices into tmp (WRITE = true).
 template <bool WRITE>
-__global__ __launch_bounds__(256) void k_bp_rows(BpRows a)
+__global__ __launch_bounds__(256) void k_bp_find(BpFind a)
 {
     __shared__ FloorDev fl;
     if (threadIdx.x == 0) fl = *a.fl;
     __syncthreads();
     const int nb = a.nb, ns = a.ns, n = nb + ns;
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= n) return;
-    const bool cube = i < nb;
-    const V3 pi = a.B.get3(F_PX, i);
-    const float ri = a.brad[i];
-    const int64_t ncc = tri(nb);
-    const int64_t base_cs = ncc + nb + ns, base_ss = base_cs + (int64_t)nb * ns;
-    const int idx_a = cube ? i : 3 * nb + ns + (i - nb);      // CC row / SS row
-    const int idx_b = 2 * nb + ns + i;                        // CS row (boxes)
-    const int idx_f = cube ? nb + i : 2 * nb + (i - nb);      // CF / SF flag
-    const int oa = WRITE ? a.offsets[idx_a] : 0;
-    const int ob = WRITE && cube ? a.offsets[idx_b] : 0;
-    // start of this body's row in the canonical CC / SS triangle (the layout tri_decode reads)
-    const int64_t row_a = cube ? (int64_t)i * (2 * (int64_t)nb - i - 1) / 2
-                               : base_ss + (int64_t)(i - nb) * (2 * (int64_t)ns - (i - nb) - 1) / 2;
-    int na = 0, nbr = 0;
-    auto emit = [&](int dst, int64_t p) {
-        if (WRITE && dst < a.cap) a.cand[dst] = (int)p;
-    };
-    auto visit = [&](int j) {
-        if (j <= i && !(cube && j >= nb)) return;  // rows hold j > i, except a box's CS row
-        const V3 pj = a.B.get3(F_PX, j);
-        if (!cull_keep(pi, ri, pj, a.brad[j])) return;
-        if (cube && j >= nb) {
-            emit(ob + nbr, base_cs + (int64_t)i * ns + (j - nb));
-            ++nbr;
+    // one candidate partner per lane (j < 0: none)
+    auto take = [&](int j) {
+        if (j < 0) return;
+        const BpPair q = bp_pair(i, j, nb, ns);
+        if (!cull_keep(a.B.get3(F_PX, q.a), a.brad[q.a], a.B.get3(F_PX, q.b), a.brad[q.b])) return;
+        if (!WRITE) {
+            atomicAdd(&a.counts[q.row], 1);
         } else {
-            emit(oa + na, row_a + (j - i - 1));
-            ++na;
+            const int dst = a.offsets[q.row] + atomicSub(&a.counts[q.row], 1) - 1;
+            if (dst < a.cap) a.tmp[dst] = (int)q.p;
         }
     };
+    const int s = a.stride;
     if (a.hslot[i] < 0) {
-        for (int j = 0; j < n; ++j) visit(j);
-    } else {
-        const int s = a.stride;
-        const long long c0 = a.cell[i], c1 = a.cell[s + i], c2 = a.cell[2 * s + i];
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const long long x = c0 + dx, y = c1 + dy, z = c2 + dz;
-                    const unsigned h = bp_hash(x, y, z, a.mask);
-                    const int e = a.bucket_off[h] + a.bucket_cnt[h];
-                    for (int k = a.bucket_off[h]; k < e; ++k) {
-                        const int j = a.sorted[k];
-                        if (a.cell[j] == x && a.cell[s + j] == y && a.cell[2 * s + j] == z) visit(j);
-                    }
-                }
-        const int nw = a.g->nwild;
-        for (int k = 0; k < nw; ++k) visit(a.wild[k]);
-    }
-    const V3 fc = mk3(fl.c[0], fl.c[1], fl.c[2]);
-    const bool fkeep = cull_keep(pi, ri, fc, fl.brad);
-    if (!WRITE) {
-        a.counts[idx_a] = na;
-        if (cube) a.counts[idx_b] = nbr;
-        a.counts[idx_f] = fkeep ? 1 : 0;
-        return;
-    }
-    if (fkeep) emit(a.offsets[idx_f], cube ? ncc + i : ncc + nb + (i - nb));
-    auto sort_row = [&](int o, int m) {
-        const int e = min(o + m, a.cap);
-        for (int k = o + 1; k < e; ++k) {
-            const int v = a.cand[k];
-            int q = k - 1;
-            while (q >= o && a.cand[q] > v) {
-                a.cand[q + 1] = a.cand[q];
-                --q;
-            }
-            a.cand[q + 1] = v;
+        for (int base = 0; base < n; base += 64) {
+            const int j = base + lane;
+            take(j < n && j != i && (a.hslot[j] >= 0 || j > i) ? j : -1);
         }
-    };
-    sort_row(oa, na);
-    if (cube) sort_row(ob, nbr);
+    } else {
+        const V3 pi = a.B.get3(F_PX, i);
+        const int ki = a.hslot[2 * s + i];
+        unsigned long long lv = (a.g->levels >> ki) << ki;
+        while (lv) {
+            const int k = __ffsll((long long)lv) - 1;
+            lv &= lv - 1ull;
+            const double inv = ldexp(a.g->inv0, -k);
+            long long x = 0, y = 0, z = 0;
+            int rs = 0, rc = 0;
+            if (lane < 27) {
+                x = bp_coord(pi.x, inv) + (lane % 3) - 1;
+                y = bp_coord(pi.y, inv) + (lane / 3) % 3 - 1;
+                z = bp_coord(pi.z, inv) + lane / 9 - 1;
+                const unsigned h = bp_hash(k, x, y, z, a.mask);
+                rs = a.bucket_off[h];
+                rc = a.bucket_cnt[h];
+            }
+            int incl = rc;  // inclusive scan of the cells' entry counts over the lanes
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int t = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += t;
+            }
+            const int total = __shfl(incl, 63, 64);
+            for (int base = 0; base < total; base += 64) {
+                const int t = base + lane;
+                int owner = 0;  // the cell lane holding entry t
+                for (int q = 0; q < 27; ++q) owner += __shfl(incl, q, 64) <= t ? 1 : 0;
+                owner = min(owner, 26);
+                const int ostart = __shfl(incl - rc, owner, 64), ors = __shfl(rs, owner, 64);
+                const long long ox = __shfl(x, owner, 64), oy = __shfl(y, owner, 64), oz = __shfl(z, owner, 64);
+                int j = -1;
+                if (t < total) {
+                    const int c = a.sorted[ors + (t - ostart)];
+                    if (a.hslot[2 * s + c] == k && (k > ki || c > i) && a.cell[c] == ox && a.cell[s + c] == oy &&
+                        a.cell[2 * s + c] == oz)
+                        j = c;
+                }
+                take(j);
+            }
+        }
+    }
+    if (lane != 0) return;
+    const bool cube = i < nb;
+    const int idx_f = cube ? nb + i : 2 * nb + (i - nb);  // CF / SF flag
+    const bool fkeep = cull_keep(a.B.get3(F_PX, i), a.brad[i], mk3(fl.c[0], fl.c[1], fl.c[2]), fl.brad);
+    if (!WRITE) {
+        a.counts[idx_f] = fkeep ? 1 : 0;
+    } else if (fkeep) {
+        const int dst = a.offsets[idx_f];
+        if (dst < a.cap) a.cand[dst] = (int)(tri(nb) + (cube ? i : nb + (i - nb)));
+    }
+}
+
+// Sorts every CC / CS / SS row from tmp into cand, one wave per row (rank sort:
+// the entries of a row are distinct).
+__global__ __launch_bounds__(256) void k_bp_sort(int nb, int ns, const int *offsets, const int *ncand, const int *tmp,
+                                                 int *cand, int cap)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= 2 * nb + ns) return;
+    const int idx = r < nb ? r : (r < 2 * nb ? 2 * nb + ns + (r - nb) : 3 * nb + ns + (r - 2 * nb));
+    const int L = 3 * nb + 2 * ns;
+    const int o = offsets[idx];
+    const int end = min(idx + 1 < L ? offsets[idx + 1] : *ncand, cap);
+    const int e = end - o;
+    for (int c = 0; c < e; c += 64) {
+        const int k = c + lane;
+        const int v = k < e ? tmp[o + k] : INT_MAX;
+        int rank = 0;
+        for (int d = 0; d < e; d += 64) {
+            const int w = d + lane < e ? tmp[o + d + lane] : INT_MAX;
+            for (int q = 0; q < 64; ++q) rank += __shfl(w, q, 64) < v ? 1 : 0;
+        }
+        if (k < e) cand[o + rank] = v;
+    }
+}
+
+// Exclusive scan of n counts by one workgroup, each thread summing a contiguous
+// run of up to PER values held in registers (one round of loads instead of one
+// per 1024 values).  With total != nullptr it records the candidate count and its
+// overflow like k_scan.
+template <int PER>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_regs(const int *counts, int n, int *offsets, int *total,
+                                                            int cap, unsigned long long *cnt)
+{
+    __shared__ int wave_tot[SCAN_THREADS / 64];
+    const int per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
+    const int b = threadIdx.x * per;
+    int v[PER];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        v[k] = k < per && b + k < n ? counts[b + k] : 0;
+        sum += v[k];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wave_tot[w] = x;
+    __syncthreads();
+    int woff = 0, tot = 0;
+    for (int k = 0; k < SCAN_THREADS / 64; ++k) {
+        const int t = wave_tot[k];
+        woff += k < w ? t : 0;
+        tot += t;
+    }
+    int run = woff + x - sum;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (k < per && b + k < n) offsets[b + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0 && total) {
+        *total = min(tot, cap);
+        atomicAdd(&cnt[CNT_CANDIDATES], (unsigned long long)tot);
+        if (tot > cap) atomicAdd(&cnt[CNT_CAND_OVERFLOW], 1ull);
+    }
+}
+
+// k_scan_regs for n <= 48 * SCAN_THREADS, k_scan beyond.
+inline void scan_counts(const int *counts, int n, int *offsets, int *total, int cap, unsigned long long *cnt,
+                        hipStream_t s)
+{
+    if (n <= 16 * SCAN_THREADS)
+        k_scan_regs<16><<<1, SCAN_THREADS, 0, s>>>(counts, n, offsets, total, cap, cnt);
+    else if (n <= 48 * SCAN_THREADS)
+        k_scan_regs<48><<<1, SCAN_THREADS, 0, s>>>(counts, n, offsets, total, cap, cnt);
+    else
+        k_scan<<<1, SCAN_THREADS, 0, s>>>(counts, n, offsets, total, cap, cnt);
 }
 
 }  // namespace nans

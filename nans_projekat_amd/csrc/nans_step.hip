@@ -617,9 +617,9 @@ int worlds_release(nans_ctx *ctx)
                     (void *)W->cand, (void *)W->rec, (void *)W->flag, (void *)W->con, (void *)W->done,
                     (void *)W->qidx, (void *)W->qsim,
                     (void *)W->blk_counts, (void *)W->blk_offsets, (void *)W->ncand, (void *)W->queue,
-                    (void *)W->bp_cell, (void *)W->bp_hslot, (void *)W->bp_sorted, (void *)W->bp_wild,
+                    (void *)W->bp_cell, (void *)W->bp_hslot, (void *)W->bp_sorted,
                     (void *)W->bp_bcnt, (void *)W->bp_boff, (void *)W->bp_counts, (void *)W->bp_offsets,
-                    W->bp_grid})
+                    (void *)W->bp_tmp, W->bp_grid})
         wfree(p);
     for (hipEvent_t ev : {W->ev_in, W->ev_out, W->ev_t0, W->ev_t1})
         if (ev) hipEventDestroy(ev);
@@ -712,41 +712,44 @@ static bool use_grid(const nans_ctx *ctx, int n)
     return n >= GRID_MIN_BODIES;
 }
 
-// Candidates of one world from the uniform grid (nans_broadphase.hpp), in
+// Candidates of one world from the hierarchical grid (nans_broadphase.hpp), in
 // canonical order, into W->cand / W->ncand.
 static void grid_broadphase(nans_worlds *W, const Planes &B, const float *brad, int nb, int ns, int64_t npairs,
                             hipStream_t s)
 {
-    const int n = nb + ns, blocks = (n + 255) / 256;
+    const int n = nb + ns, blocks = (n + 255) / 256, L = 3 * nb + 2 * ns;
     BpGrid *g = (BpGrid *)W->bp_grid;
     const unsigned mask = (unsigned)W->bp_buckets - 1;
-    k_bp_bounds<<<1, BP_THREADS, 0, s>>>(B, brad, n, W->bp_bcnt, W->bp_buckets, g, W->bp_wild, W->cnt,
+    k_bp_bounds<<<1, BP_THREADS, 0, s>>>(B, brad, n, W->bp_bcnt, W->bp_buckets, W->bp_counts, L, g, W->cnt,
                                           (long long)npairs);
     k_bp_cells<<<blocks, 256, 0, s>>>(B, brad, n, g, mask, W->bp_cell, W->nmax, W->bp_hslot, W->bp_bcnt);
-    k_scan<<<1, SCAN_THREADS, 0, s>>>(W->bp_bcnt, W->bp_buckets, W->bp_boff, nullptr, 0, nullptr);
+    scan_counts(W->bp_bcnt, W->bp_buckets, W->bp_boff, nullptr, 0, nullptr, s);
     k_bp_scatter<<<blocks, 256, 0, s>>>(n, W->bp_hslot, W->nmax, W->bp_boff, W->bp_sorted);
-    BpRows r;
-    r.B = B;
-    r.brad = brad;
-    r.nb = nb;
-    r.ns = ns;
-    r.fl = W->floor;
-    r.g = g;
-    r.mask = mask;
-    r.cell = W->bp_cell;
-    r.stride = W->nmax;
-    r.hslot = W->bp_hslot;
-    r.bucket_cnt = W->bp_bcnt;
-    r.bucket_off = W->bp_boff;
-    r.sorted = W->bp_sorted;
-    r.wild = W->bp_wild;
-    r.counts = W->bp_counts;
-    r.offsets = W->bp_offsets;
-    r.cand = W->cand;
-    r.cap = W->cand_cap;
-    k_bp_rows<false><<<blocks, 256, 0, s>>>(r);
-    k_scan<<<1, SCAN_THREADS, 0, s>>>(W->bp_counts, 3 * nb + 2 * ns, W->bp_offsets, W->ncand, W->cand_cap, W->cnt);
-    k_bp_rows<true><<<blocks, 256, 0, s>>>(r);
+    BpFind f;
+    f.B = B;
+    f.brad = brad;
+    f.nb = nb;
+    f.ns = ns;
+    f.fl = W->floor;
+    f.g = g;
+    f.mask = mask;
+    f.cell = W->bp_cell;
+    f.stride = W->nmax;
+    f.hslot = W->bp_hslot;
+    f.bucket_cnt = W->bp_bcnt;
+    f.bucket_off = W->bp_boff;
+    f.sorted = W->bp_sorted;
+    f.counts = W->bp_counts;
+    f.offsets = W->bp_offsets;
+    f.tmp = W->bp_tmp;
+    f.cand = W->cand;
+    f.cap = W->cand_cap;
+    const int wave_blocks = (n + 3) / 4;  // one wave per body
+    k_bp_find<false><<<wave_blocks, 256, 0, s>>>(f);
+    scan_counts(W->bp_counts, L, W->bp_offsets, W->ncand, W->cand_cap, W->cnt, s);
+    k_bp_find<true><<<wave_blocks, 256, 0, s>>>(f);
+    k_bp_sort<<<(2 * nb + ns + 3) / 4, 256, 0, s>>>(nb, ns, W->bp_offsets, W->ncand, W->bp_tmp, W->cand,
+                                                    W->cand_cap);
 }
 
 }  // namespace nans
@@ -856,13 +859,13 @@ int nans_upload_worlds(nans_ctx *ctx, int32_t nworlds, const int32_t *world_offs
     W->bp_buckets = 64;
     while (W->bp_buckets < 2 * nmax) W->bp_buckets *= 2;
     NANS_TRY(dmalloc(&W->bp_cell, (size_t)3 * nmax));
-    NANS_TRY(dmalloc(&W->bp_hslot, (size_t)2 * nmax));
+    NANS_TRY(dmalloc(&W->bp_hslot, (size_t)3 * nmax));
     NANS_TRY(dmalloc(&W->bp_sorted, (size_t)nmax));
-    NANS_TRY(dmalloc(&W->bp_wild, (size_t)nmax));
     NANS_TRY(dmalloc(&W->bp_bcnt, (size_t)W->bp_buckets));
     NANS_TRY(dmalloc(&W->bp_boff, (size_t)W->bp_buckets));
     NANS_TRY(dmalloc(&W->bp_counts, (size_t)3 * nmax));
     NANS_TRY(dmalloc(&W->bp_offsets, (size_t)3 * nmax));
+    NANS_TRY(dmalloc(&W->bp_tmp, (size_t)cap));
     NANS_TRY(dmalloc((BpGrid **)&W->bp_grid, 1));
     NANS_HIP(hipFuncSetAttribute((const void *)k_plain_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)(sizeof(int) * PLAIN_MAX_BODIES)));
@@ -1005,8 +1008,7 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
                 } else {
                     k_plain_cull<false><<<nblk, 256, 0, s>>>(B, brad, nb, ns, npairs, W->floor, W->blk_counts,
                                                              nullptr, nullptr, 0, W->cnt);
-                    k_scan<<<1, SCAN_THREADS, 0, s>>>(W->blk_counts, nblk, W->blk_offsets, W->ncand, W->cand_cap,
-                                                      W->cnt);
+                    scan_counts(W->blk_counts, nblk, W->blk_offsets, W->ncand, W->cand_cap, W->cnt, s);
                     k_plain_cull<true><<<nblk, 256, 0, s>>>(B, brad, nb, ns, npairs, W->floor, nullptr,
                                                             W->blk_offsets, W->cand, W->cand_cap, W->cnt);
                 }

@@ -1,10 +1,11 @@
 """CPU check of the grid broadphase's soundness argument
 (nans_projekat_amd/csrc/nans_broadphase.hpp): whenever the plain step's cull test
-keeps a pair of tame bodies, their grid cells differ by at most one per axis, so
-the 27-cell search finds the pair.  Pairs are placed on the test's boundary (just
-inside, on, and just outside the limit) over radii and centre magnitudes from
-1e-3 to 1e17, with the cull test and the cell coordinates evaluated in the
-device's arithmetic (binary32 without contraction; cells in binary64)."""
+keeps a pair of tame bodies, their distance per axis is within the sum of their
+reaches, and their cells at the coarser body's level differ by at most one per
+axis, so the 27-cell search finds the pair.  Pairs are placed on the test's
+boundary (just inside, on, and just outside the limit) over radii from 1e-3 to
+1e14 and centre magnitudes from 1e-3 to 1e14, with the cull test in the device's
+binary32 arithmetic (no contraction) and reaches, levels and cells in binary64."""
 import numpy as np
 
 F = np.float32
@@ -20,16 +21,34 @@ def cull_keep(pa, ra, pb, rb):
     return ~(dist2 > lim * lim), lim
 
 
-def cell_inv(r, m):
-    """1 / cell size of k_bp_bounds for the largest tame |radius| r and |x|+|y|+|z| m."""
-    R, M = np.float64(r), np.float64(m)
-    lim = 2.0 * R * 1.0001 + 1e-4 * (1.0 + 2.0 * M + 2.0 * R)
-    return 1.0 / (lim * 1.01)
+def reach(p, r):
+    """Per-body reach of the grid (k_bp_cells), binary64 from binary32 inputs."""
+    mag = (np.abs(p[:, 0]) + np.abs(p[:, 1])) + np.abs(p[:, 2])
+    return (np.abs(r).astype(np.float64) * 1.0002 + 1e-4 * (0.5 + mag.astype(np.float64))) * (1.0 + 4e-6)
+
+
+def rho0_of(rmax):
+    return (np.float64(rmax) * 1.0002 + 1e-4 * (0.5 + 128.0)) * (1.0 + 4e-6)
+
+
+def level(rho, rho0):
+    k = np.zeros(rho.shape, np.int64)
+    lim = np.full(rho.shape, rho0, np.float64)
+    for _ in range(63):
+        up = rho > lim
+        k += up
+        lim = np.where(up, lim * 2.0, lim)
+    return k
 
 
 def coord(x, inv):
     u = np.floor(x.astype(np.float64) * inv)
     return np.clip(u, -2.0 ** 40, 2.0 ** 40).astype(np.int64)
+
+
+def cells_at(p, k, rho0):
+    inv = np.ldexp(1.0 / (2.02 * rho0), -k)
+    return np.stack([coord(p[:, c], inv) for c in range(3)], axis=1)
 
 
 def _boundary_pairs(rng, n, scale_r, scale_p):
@@ -50,19 +69,23 @@ def test_kept_pairs_lie_in_neighbouring_cells():
     rng = np.random.default_rng(11)
     checked = 0
     for scale_r in (1e-3, 0.1, 1.0, 30.0, 1e4, 1e8, 1e14):
-        for scale_p in (1e-3, 1.0, 1e3, 1e6, 1e10, 1e15, 1e17):
+        for scale_p in (1e-3, 1.0, 1e3, 1e6, 1e10, 1e14):
             pa, ra, pb, rb = _boundary_pairs(rng, 20000, scale_r, scale_p)
             keep, _ = cull_keep(pa, ra, pb, rb)
             mag = lambda p: (np.abs(p[:, 0]) + np.abs(p[:, 1])) + np.abs(p[:, 2])
-            tame = (mag(pa) <= F(1e18)) & (mag(pb) <= F(1e18))
+            tame = (mag(pa) <= F(1e15)) & (mag(pb) <= F(1e15)) & (np.abs(ra) <= F(1e15)) & (np.abs(rb) <= F(1e15))
             sel = keep & tame
-            # the tightest grid: a world holding just the two bodies of each pair
-            inv = cell_inv(np.maximum(np.abs(ra), np.abs(rb))[sel],
-                           np.maximum(mag(pa), mag(pb))[sel])
-            for c in range(3):
-                da = coord(pa[sel, c], inv)
-                db = coord(pb[sel, c], inv)
-                assert np.all(np.abs(da - db) <= 1), (scale_r, scale_p, c)
+            pa, ra, pb, rb = pa[sel], ra[sel], pb[sel], rb[sel]
+            rha, rhb = reach(pa, ra), reach(pb, rb)
+            # the reach bound the proof rests on
+            d = np.abs(pa.astype(np.float64) - pb.astype(np.float64)).max(axis=1)
+            assert np.all(d <= rha + rhb), (scale_r, scale_p)
+            # the tightest grid: rho0 from the pair's own radii, either body the finer one
+            for rho0 in (rho0_of(np.maximum(np.abs(ra), np.abs(rb)).min()), rho0_of(0.0)):
+                ka, kb = level(rha, rho0), level(rhb, rho0)
+                k = np.maximum(ka, kb)
+                assert np.all(rha <= rho0 * 2.0 ** ka) and np.all(rhb <= rho0 * 2.0 ** kb)
+                assert np.all(np.abs(cells_at(pa, k, rho0) - cells_at(pb, k, rho0)) <= 1), (scale_r, scale_p)
             checked += int(sel.sum())
     assert checked > 300000
 
@@ -76,7 +99,8 @@ def test_the_boundary_is_exercised():
 
 
 def test_far_cells_are_clamped_together():
-    inv = cell_inv(1.0, 1e17)
-    x = np.array([1e17, 1e17 * (1 + 1e-6), -1e17], F)
-    c = coord(x, inv)
-    assert abs(int(c[0]) - int(c[1])) <= 1 and c[2] < 0
+    rho0 = rho0_of(1.0)
+    k = np.array([0, 0, 0])
+    x = np.array([[1e14, 0, 0], [1e14 * (1 + 1e-6), 0, 0], [-1e14, 0, 0]], F)
+    c = cells_at(x, k, rho0)[:, 0]
+    assert abs(int(c[0]) - int(c[1])) <= 1 and c[0] == 2 ** 40 and c[2] == -2 ** 40
