@@ -331,7 +331,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int *counts, int nb
 #include "nans_broadphase.hpp"
 namespace nans {
 
-constexpr int NARROW_RANGE = 512;  // candidates per workgroup of the plain narrowphase
+constexpr int NARROW_RANGE = 512;  // largest share of candidates per workgroup of the plain narrowphase
 
 __global__ __launch_bounds__(256) void k_plain_narrow(Planes B, const float *verts, int nb, int ns,
                                                       const FloorDev *flp, const int *cand, const int *ncand,
@@ -344,8 +344,10 @@ __global__ __launch_bounds__(256) void k_plain_narrow(Planes B, const float *ver
     if (threadIdx.x == 0) fl = *flp;
     __syncthreads();
     const int total = *ncand;
-    for (int base = blockIdx.x * NARROW_RANGE; base < total; base += gridDim.x * NARROW_RANGE) {
-        const int count = min(NARROW_RANGE, total - base);
+    // shares of 64..NARROW_RANGE candidates, small enough to give every workgroup one
+    const int range = min(NARROW_RANGE, max(64, ((total + (int)gridDim.x - 1) / (int)gridDim.x + 63) & ~63));
+    for (int base = blockIdx.x * range; base < total; base += gridDim.x * range) {
+        const int count = min(range, total - base);
         WorldSrc src{B, verts, nb, ns, &fl, cand + base, rec + base, flag + base, cnt};
         narrow_block<256>(src, count, np, lds, sh, qidx + base, qsim + base);
         if (threadIdx.x == 0) atomicAdd(&cnt[CNT_GJK_FOUND], (unsigned long long)sh.nepa);
