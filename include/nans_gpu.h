@@ -209,6 +209,25 @@ int nans_batched_timing(nans_ctx *ctx, nans_timing *timing);
  * uploaded batch in place, keeping buffers and the captured graph. */
 int nans_set_state(nans_ctx *ctx, const nans_bodies_soa *bodies);
 
+/* Orientation of both steps.  EULER (the default) is the reference's: Angles in
+ * degrees advanced by Angles += dt * W (code/nans.cpp:1339) and read by the Model
+ * rebuild as translate * rotX * rotY * rotZ * scale (code/nans.cpp:1913-1924).
+ * QUATERNION is an opt-in mode with no counterpart in the reference (its README
+ * lists quaternions as a TODO): each body carries a unit quaternion q = (w, x, y,
+ * z), the box Model is [R(q) diag(scale) | pos], and IntegrateVelocities advances
+ * q <- normalize(q + dt/2 (0, W) q), W read as the world-frame angular velocity in
+ * rad/s (as the Constraint terms W x N read it).  Results differ from the
+ * reference's by design.  Switching to QUATERNION takes each body's quaternion
+ * from its current angles (R(q) = Rx Ry Rz); the angles are then left as they are.
+ * nans_upload_worlds sets the quaternions from the uploaded angles;
+ * nans_set_quaternions / nans_get_quaternions move them as [4][N] planes (w, x, y,
+ * z).  The differentiable rollout requires EULER. */
+#define NANS_ORIENT_EULER 0
+#define NANS_ORIENT_QUATERNION 1
+int nans_set_orientation_mode(nans_ctx *ctx, int32_t mode);
+int nans_set_quaternions(nans_ctx *ctx, const float *quat);
+int nans_get_quaternions(nans_ctx *ctx, float *quat);
+
 /* Broadphase of the plain step.  The reference tests every pair of a world
  * (code/nans.cpp:1355-1535); the plain step keeps the pairs whose bounding
  * spheres can touch, found either by testing every canonical pair or with a

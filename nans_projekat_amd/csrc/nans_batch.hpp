@@ -73,7 +73,7 @@ inline int dmalloc(T **p, size_t n)
     } while (0)
 
 void wfree(void *p);
-StepConsts step_consts(const nans_params &p);
+StepConsts step_consts(const nans_params &p, int orientation);
 int setup_floor(nans_ctx *ctx, hipStream_t s);
 int check_overflow(nans_ctx *ctx);
 // nsteps batched steps replayed from the captured one-step graph
@@ -153,6 +153,7 @@ struct nans_worlds {
     int graph_steps = -1;
     int graph_block = 0;
     nans_params graph_params;
+    int graph_orientation = 0;
     hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
     int graph_grid = 0, graph_lds = 0;
     int timed_steps = 0;

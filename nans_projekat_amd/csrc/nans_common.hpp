@@ -62,4 +62,5 @@ struct nans_ctx {
     int *np_qidx;                    // queued GJK hits
     void *np_qsim;                   // their simplices (SimplexRec)
     int broadphase;                  // NANS_BROADPHASE_* of the plain step
+    int orientation;                 // NANS_ORIENT_* of both steps
 };

@@ -150,6 +150,33 @@ __device__ inline void model_vertices(const M4 &m, V3 *verts)
     }
 }
 
+// Opt-in quaternion orientation of the step (no counterpart in the reference,
+// whose Angles are Euler degrees): q = (w, x, y, z), Model = [R(q) diag(S) | P].
+__device__ inline M4 model_from_quat(V3 pos, float w, float x, float y, float z, V3 scale)
+{
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float xy = x * y, xz = x * z, yz = y * z;
+    const float wx = w * x, wy = w * y, wz = w * z;
+    M4 m;
+    m.m[0] = (1.0f - 2.0f * (yy + zz)) * scale.x;
+    m.m[1] = (2.0f * (xy + wz)) * scale.x;
+    m.m[2] = (2.0f * (xz - wy)) * scale.x;
+    m.m[3] = 0.0f;
+    m.m[4] = (2.0f * (xy - wz)) * scale.y;
+    m.m[5] = (1.0f - 2.0f * (xx + zz)) * scale.y;
+    m.m[6] = (2.0f * (yz + wx)) * scale.y;
+    m.m[7] = 0.0f;
+    m.m[8] = (2.0f * (xz + wy)) * scale.z;
+    m.m[9] = (2.0f * (yz - wx)) * scale.z;
+    m.m[10] = (1.0f - 2.0f * (xx + yy)) * scale.z;
+    m.m[11] = 0.0f;
+    m.m[12] = pos.x;
+    m.m[13] = pos.y;
+    m.m[14] = pos.z;
+    m.m[15] = 1.0f;
+    return m;
+}
+
 // Box corners from (centre, unit quaternion (w,x,y,z), half-extents): Model columns
 // R(q)[:,k] * (2 h_k) and (c, 1), applied with the glm mat4*vec4 form.  Used by the
 // narrowphase entry point, whose shapes carry quaternions.

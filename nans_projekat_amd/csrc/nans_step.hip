@@ -251,7 +251,8 @@ __global__ __launch_bounds__(256) void k_plain_forces(Planes B, float *verts, fl
     integrate_forces_body(B, i, F, T, c);
     B.set3(F_FX, i, mk3(0.0f, 0.0f, 0.0f));
     B.set3(F_TX, i, mk3(0.0f, 0.0f, 0.0f));
-    brad[i] = i < nb ? box_vertices_body(B, i, verts + 24 * i) : B.at(F_SX, i);
+    brad[i] = i >= nb ? B.at(F_SX, i)
+                      : (c.quat ? box_vertices_quat(B, B, i, verts + 24 * i) : box_vertices_body(B, i, verts + 24 * i));
 }
 
 template <bool WRITE>
@@ -398,7 +399,10 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
     if (threadIdx.x == 0) atomicAdd(&a.cnt[CNT_CONTACTS], (unsigned long long)ncon);
     __syncthreads();
     solve_rounds<SOLVE_THREADS>(a.B, a.n, a.con, ncon, a.done, head, fl, a.k, a.Cf, a.Cr);
-    for (int i = threadIdx.x; i < a.n; i += SOLVE_THREADS) integrate_velocities_body(a.B, i, a.k.dt);
+    for (int i = threadIdx.x; i < a.n; i += SOLVE_THREADS) {
+        if (a.k.quat) integrate_velocities_quat(a.B, a.B, i, a.k.dt);
+        else integrate_velocities_body(a.B, i, a.k.dt);
+    }
     if (a.rec_out) {
         __syncthreads();
         for (int c = threadIdx.x; c < ncon && c < a.rec_cap; c += SOLVE_THREADS) a.rec_out[c] = a.con[c];
@@ -458,7 +462,9 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgs a)
             integrate_forces_body(Lb, i, F, T, a.k);
             Gw.set3(F_FX, i, mk3(0.0f, 0.0f, 0.0f));
             Gw.set3(F_TX, i, mk3(0.0f, 0.0f, 0.0f));
-            bradL[i] = i < nb ? box_vertices_body(Lb, i, vertsL + 24 * i) : Lb.at(F_SX, i);
+            bradL[i] = i >= nb ? Lb.at(F_SX, i)
+                               : (a.k.quat ? box_vertices_quat(Lb, Gw, i, vertsL + 24 * i)
+                                           : box_vertices_body(Lb, i, vertsL + 24 * i));
         }
         __syncthreads();
         // cull over the canonical pairs, in canonical order: wave w takes the w-th
@@ -538,7 +544,8 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgs a)
                             (int)(NW * sizeof(EpaLds) / sizeof(ContactTerms)));
         // IntegrateVelocities and write-back
         for (int i = tid; i < n; i += BLOCK) {
-            integrate_velocities_body(Lb, i, a.k.dt);
+            if (a.k.quat) integrate_velocities_quat(Lb, Gw, i, a.k.dt);
+            else integrate_velocities_body(Lb, i, a.k.dt);
 #pragma unroll
             for (int f = 0; f < F_SX; ++f) Gw.at(f, i) = Lb.at(f, i);
         }
@@ -633,9 +640,26 @@ int worlds_release(nans_ctx *ctx)
     return NANS_OK;
 }
 
-StepConsts step_consts(const nans_params &p)
+// Quaternions (w, x, y, z planes) of Euler angles in degrees ([3][n] planes),
+// composed as the Model rebuild composes them: q = qx(ax) qy(ay) qz(az), so that
+// R(q) = Rx Ry Rz (code/nans.cpp:1913-1924).  Binary64, rounded once.
+static void euler_to_quat_planes(const float *ang, float *q, int64_t n)
+{
+    const double k = 3.14159265358979323846 / 360.0;  // degrees to half-angle radians
+    for (int64_t i = 0; i < n; ++i) {
+        const double hx = ang[i] * k, hy = ang[n + i] * k, hz = ang[2 * n + i] * k;
+        const double cx = cos(hx), sx = sin(hx), cy = cos(hy), sy = sin(hy), cz = cos(hz), sz = sin(hz);
+        q[i] = (float)(cx * cy * cz - sx * sy * sz);
+        q[n + i] = (float)(sx * cy * cz + cx * sy * sz);
+        q[2 * n + i] = (float)(cx * sy * cz - sx * cy * sz);
+        q[3 * n + i] = (float)(cx * cy * sz + sx * sy * cz);
+    }
+}
+
+StepConsts step_consts(const nans_params &p, int orientation)
 {
     StepConsts k;
+    k.quat = orientation == NANS_ORIENT_QUATERNION;
     k.np = narrow_params(p);
     k.solver_iterations = p.solver_iterations;
     k.beta = p.beta;
@@ -823,6 +847,7 @@ int nans_upload_worlds(nans_ctx *ctx, int32_t nworlds, const int32_t *world_offs
     put3(F_TX, b->torque);
     std::copy(b->mass, b->mass + N, h.begin() + (size_t)F_MASS * N);
     std::copy(b->moi, b->moi + N, h.begin() + (size_t)F_MOI * N);
+    euler_to_quat_planes(h.data() + (size_t)F_AX * N, h.data() + (size_t)F_QW * N, N);
     NANS_HIP(hipMemcpy(W->planes, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
     NANS_HIP(hipMemcpy(W->offsets, off.data(), sizeof(int) * (nworlds + 1), hipMemcpyHostToDevice));
     NANS_HIP(hipMemcpy(W->nboxes, nbx.data(), sizeof(int) * nworlds, hipMemcpyHostToDevice));
@@ -993,7 +1018,7 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
     NANS_HIP(hipSetDevice(ctx->p.device));
     hipStream_t s = ctx->stream;
     NANS_TRY(setup_floor(ctx, s));
-    const StepConsts k = step_consts(ctx->p);
+    const StepConsts k = step_consts(ctx->p, ctx->orientation);
     for (int step = 0; step < nsteps; ++step) {
         for (int w = 0; w < W->nworlds; ++w) {
             const int b0 = W->h_off[w], n = W->h_off[w + 1] - b0, nb = W->h_nb[w], ns = n - nb;
@@ -1097,7 +1122,7 @@ static int batched_prepare(nans_ctx *ctx, BatchArgs &a, int *grid_out, size_t *l
     a.order = W->order;
     a.nworlds = W->nworlds;
     a.nmax = W->nmax;
-    a.k = step_consts(ctx->p);
+    a.k = step_consts(ctx->p, ctx->orientation);
     a.fl = W->floor;
     a.wfric = W->wfric;
     a.wrest = W->wrest;
@@ -1141,6 +1166,7 @@ static int batched_capture(nans_ctx *ctx, int nsteps, hipStream_t cs)
     W->graph_steps = nsteps;
     W->graph_block = BLOCK;
     W->graph_params = ctx->p;
+    W->graph_orientation = ctx->orientation;
     W->graph_grid = grid;
     W->graph_lds = (int)lds;
     return NANS_OK;
@@ -1159,7 +1185,8 @@ int batched_run(nans_ctx *ctx, int nsteps, int block)
         NANS_HIP(hipEventCreate(&W->ev_t0));
         NANS_HIP(hipEventCreate(&W->ev_t1));
     }
-    bool reuse = W->graph && W->graph_block == block && memcmp(&W->graph_params, &ctx->p, sizeof(nans_params)) == 0;
+    bool reuse = W->graph && W->graph_block == block && W->graph_orientation == ctx->orientation &&
+                 memcmp(&W->graph_params, &ctx->p, sizeof(nans_params)) == 0;
     if (!reuse) {
         NANS_TRY(setup_floor(ctx, ctx->stream));
         NANS_HIP(hipStreamSynchronize(ctx->stream));
@@ -1323,6 +1350,49 @@ int nans_autotune_batched(nans_ctx *ctx, int32_t nsteps, nans_tuning *out)
         out->threads_per_world = best;
         out->steps = nsteps;
     }
+    return NANS_OK;
+}
+
+int nans_set_orientation_mode(nans_ctx *ctx, int32_t mode)
+{
+    NANS_REQUIRE(ctx, NANS_ERR_INVALID, "ctx is NULL");
+    NANS_REQUIRE(mode == NANS_ORIENT_EULER || mode == NANS_ORIENT_QUATERNION, NANS_ERR_INVALID,
+                 "orientation mode %d is not one of NANS_ORIENT_*", (int)mode);
+    nans_worlds *W = ctx->worlds;
+    if (W && mode == NANS_ORIENT_QUATERNION && ctx->orientation != mode) {
+        // the quaternions of the current angles
+        const int64_t N = W->N;
+        NANS_HIP(hipSetDevice(ctx->p.device));
+        NANS_HIP(hipStreamSynchronize(ctx->stream));
+        std::vector<float> ang((size_t)3 * N), q((size_t)4 * N);
+        NANS_HIP(hipMemcpy(ang.data(), W->planes + (size_t)F_AX * N, sizeof(float) * ang.size(),
+                           hipMemcpyDeviceToHost));
+        euler_to_quat_planes(ang.data(), q.data(), N);
+        NANS_HIP(hipMemcpy(W->planes + (size_t)F_QW * N, q.data(), sizeof(float) * q.size(), hipMemcpyHostToDevice));
+    }
+    ctx->orientation = mode;
+    return NANS_OK;
+}
+
+int nans_set_quaternions(nans_ctx *ctx, const float *quat)
+{
+    NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
+    NANS_REQUIRE(quat, NANS_ERR_INVALID, "quat is NULL");
+    nans_worlds *W = ctx->worlds;
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    NANS_HIP(hipStreamSynchronize(ctx->stream));
+    NANS_HIP(hipMemcpy(W->planes + (size_t)F_QW * W->N, quat, sizeof(float) * 4 * W->N, hipMemcpyHostToDevice));
+    return NANS_OK;
+}
+
+int nans_get_quaternions(nans_ctx *ctx, float *quat)
+{
+    NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
+    NANS_REQUIRE(quat, NANS_ERR_INVALID, "quat is NULL");
+    nans_worlds *W = ctx->worlds;
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    NANS_HIP(hipStreamSynchronize(ctx->stream));
+    NANS_HIP(hipMemcpy(quat, W->planes + (size_t)F_QW * W->N, sizeof(float) * 4 * W->N, hipMemcpyDeviceToHost));
     return NANS_OK;
 }
 

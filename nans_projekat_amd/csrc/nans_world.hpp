@@ -19,6 +19,7 @@ enum : int {
     F_SX, F_SY, F_SZ, F_MASS, F_MOI,
     F_NLDS,
     F_FX = F_NLDS, F_FY, F_FZ, F_TX, F_TY, F_TZ,
+    F_QW, F_QX, F_QY, F_QZ,  // orientation of the opt-in quaternion mode
     F_NFIELDS
 };
 
@@ -39,6 +40,7 @@ struct StepConsts {
     NarrowParams np;
     int solver_iterations;
     float beta, restitution, friction, damping, gravity, dt;
+    int quat;  // 1: orientation is the quaternion planes (opt-in), 0: Euler degrees
 };
 
 // The ground box of code/nans.cpp:1666-1678: static, its V and W read as zero.
@@ -94,14 +96,34 @@ __host__ __device__ inline void integrate_velocities_body(const Planes &B, int i
     B.set3(F_AX, i, add(B.get3(F_AX, i), mul(B.get3(F_WX, i), dt)));
 }
 
-// Box corners of the Model rebuilt from the pose (code/nans.cpp:1913-1941 then
-// UpdateVertices :394-407) and the bounding radius max |v - P| used by the cull.
-__device__ inline float box_vertices_body(const Planes &B, int i, float *out24)
+// IntegrateVelocities of the opt-in quaternion mode: P += dt V, and
+// q <- normalize(q + dt/2 (0, W) q) with W the world-frame angular velocity in
+// rad/s (the quantity the Constraint terms W x N treat it as); in exact
+// arithmetic one step rotates by 2 atan(|W| dt / 2) about W.  The quaternion
+// planes may live in another view than the pose (Q: global, B: LDS).
+__host__ __device__ inline void integrate_velocities_quat(const Planes &B, const Planes &Q, int i, float dt)
 {
-    M4 m = model_from_pose(B.get3(F_PX, i), B.get3(F_AX, i), B.get3(F_SX, i));
+    B.set3(F_PX, i, add(B.get3(F_PX, i), mul(B.get3(F_VX, i), dt)));
+    const V3 W = B.get3(F_WX, i);
+    const float h = 0.5f * dt;
+    const float w = Q.at(F_QW, i), x = Q.at(F_QX, i), y = Q.at(F_QY, i), z = Q.at(F_QZ, i);
+    const float nw = w - h * ((W.x * x + W.y * y) + W.z * z);
+    const float nx = x + h * (w * W.x + (W.y * z - W.z * y));
+    const float ny = y + h * (w * W.y + (W.z * x - W.x * z));
+    const float nz = z + h * (w * W.z + (W.x * y - W.y * x));
+    const float inv = 1.0f / sqrtf(((nw * nw + nx * nx) + ny * ny) + nz * nz);
+    Q.at(F_QW, i) = nw * inv;
+    Q.at(F_QX, i) = nx * inv;
+    Q.at(F_QY, i) = ny * inv;
+    Q.at(F_QZ, i) = nz * inv;
+}
+
+// Corners of a box Model (UpdateVertices, code/nans.cpp:394-407) and the
+// bounding radius max |v - P| used by the cull.
+__device__ inline float box_vertices_model(const M4 &m, V3 P, float *out24)
+{
     V3 v[8];
     model_vertices(m, v);
-    V3 P = B.get3(F_PX, i);
     float r2 = 0.0f;
     bool bad = false;
 #pragma unroll
@@ -116,6 +138,21 @@ __device__ inline float box_vertices_body(const Planes &B, int i, float *out24)
     }
     // a NaN corner makes the radius NaN, which keeps every pair of the body
     return bad ? __int_as_float(0x7fc00000) : sqrtf(r2);
+}
+
+// Box corners of the Model rebuilt from the pose (code/nans.cpp:1913-1941), or in
+// the quaternion mode from the quaternion planes of Q.
+__device__ inline float box_vertices_body(const Planes &B, int i, float *out24)
+{
+    const V3 P = B.get3(F_PX, i);
+    return box_vertices_model(model_from_pose(P, B.get3(F_AX, i), B.get3(F_SX, i)), P, out24);
+}
+
+__device__ inline float box_vertices_quat(const Planes &B, const Planes &Q, int i, float *out24)
+{
+    const V3 P = B.get3(F_PX, i);
+    return box_vertices_model(
+        model_from_quat(P, Q.at(F_QW, i), Q.at(F_QX, i), Q.at(F_QY, i), Q.at(F_QZ, i), B.get3(F_SX, i)), P, out24);
 }
 
 // ------------------------------------------------------------------ pairs (code/nans.cpp:1355-1535)

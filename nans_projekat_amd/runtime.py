@@ -296,6 +296,22 @@ class Context:
     def set_batched_block(self, threads_per_world: int) -> None:
         check(self.lib.nans_set_batched_block(self.h, int(threads_per_world)))
 
+    ORIENTATION = {"euler": 0, "quaternion": 1}
+
+    def set_orientation_mode(self, mode: str) -> None:
+        """"euler" (the reference's degrees) or the opt-in "quaternion" mode."""
+        check(self.lib.nans_set_orientation_mode(self.h, self.ORIENTATION[mode]))
+
+    def quaternions(self) -> np.ndarray:
+        """Orientation of the quaternion mode as [N][4] (w, x, y, z)."""
+        q = np.zeros((4, self.nbodies), np.float32)
+        check(self.lib.nans_get_quaternions(self.h, _np(q)))
+        return np.ascontiguousarray(q.T)
+
+    def set_quaternions(self, quat) -> None:
+        q = np.ascontiguousarray(np.asarray(quat, np.float32).reshape(self.nbodies, 4).T)
+        check(self.lib.nans_set_quaternions(self.h, _np(q)))
+
     BROADPHASE = {"auto": 0, "all_pairs": 1, "grid": 2}
 
     def set_broadphase(self, mode: str) -> None:

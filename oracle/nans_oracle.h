@@ -159,6 +159,25 @@ int ora_world_run_f64(int32_t nb, int32_t ns,
                       ora_contact_f64 *contacts, int32_t max_contacts, int32_t *ncontacts,
                       int64_t *stats);
 
+/* ora_world_run with the opt-in quaternion orientation (no counterpart in the
+ * reference): quat [n][4] (w, x, y, z, unit) is the orientation in and out; the box
+ * Model is [R(q) diag(scale) | pos] and IntegrateVelocities advances
+ * q <- normalize(q + dt/2 (0, angvel) q) instead of the angles (left unchanged). */
+int ora_world_run_quat_f32(int32_t nb, int32_t ns,
+                           float *pos, float *vel, float *angvel, float *angles, float *quat,
+                           float *force, float *torque,
+                           const float *scale, const float *mass, const float *moi,
+                           const ora_step_params *p, int32_t nsteps,
+                           ora_contact_f32 *contacts, int32_t max_contacts, int32_t *ncontacts,
+                           int64_t *stats);
+int ora_world_run_quat_f64(int32_t nb, int32_t ns,
+                           double *pos, double *vel, double *angvel, double *angles, double *quat,
+                           double *force, double *torque,
+                           const double *scale, const double *mass, const double *moi,
+                           const ora_step_params *p, int32_t nsteps,
+                           ora_contact_f64 *contacts, int32_t max_contacts, int32_t *ncontacts,
+                           int64_t *stats);
+
 /* Every canonical pair of the world's current pose (DetectCollisions order,
  * nans.cpp:1355-1535): type/index_a/index_b and the per-pair stats of
  * CheckCollision, without solving or stepping.  Arrays have room for
@@ -172,6 +191,9 @@ int ora_world_pairs_f32(int32_t nb, int32_t ns,
  * degrees), in the order of code/nans.cpp:399-406: out [8][3]. */
 void ora_box_vertices_f32(const float *pos, const float *angles, const float *scale, float *out);
 void ora_box_vertices_f64(const double *pos, const double *angles, const double *scale, double *out);
+/* Box corners of the quaternion-mode Model [R(q) diag(scale) | pos], quat (w, x, y, z). */
+void ora_box_vertices_quat_f32(const float *pos, const float *quat, const float *scale, float *out);
+void ora_box_vertices_quat_f64(const double *pos, const double *quat, const double *scale, double *out);
 
 /* Work counters of the step (see nans_oracle_impl.h): enable/disable (and reset),
  * optionally applying the HIP step's bounding-sphere cull; read 32 counters. */

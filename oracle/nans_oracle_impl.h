@@ -193,6 +193,54 @@ static void SUF(update_vertices)(const M4 *m, V3 *verts)
         verts[k] = SUF(m4_apply)(m, SUF(kCorner)[k][0], SUF(kCorner)[k][1], SUF(kCorner)[k][2], (REAL)1);
 }
 
+/* Opt-in quaternion orientation (no counterpart in the reference, whose Angles
+ * are Euler degrees, code/nans.cpp:1339, 1913-1924).  q = (w, x, y, z), unit.
+ * Model = [R(q) * diag(S) | P]: the columns of the rotation matrix of q scaled by
+ * the Model scale, applied to the corners like the Euler Model. */
+static M4 SUF(model_from_quat)(V3 pos, const REAL *q, V3 scale)
+{
+    const REAL w = q[0], x = q[1], y = q[2], z = q[3];
+    const REAL xx = x * x, yy = y * y, zz = z * z;
+    const REAL xy = x * y, xz = x * z, yz = y * z;
+    const REAL wx = w * x, wy = w * y, wz = w * z;
+    M4 m;
+    m.m[0] = ((REAL)1 - (REAL)2 * (yy + zz)) * scale.x;
+    m.m[1] = ((REAL)2 * (xy + wz)) * scale.x;
+    m.m[2] = ((REAL)2 * (xz - wy)) * scale.x;
+    m.m[3] = 0;
+    m.m[4] = ((REAL)2 * (xy - wz)) * scale.y;
+    m.m[5] = ((REAL)1 - (REAL)2 * (xx + zz)) * scale.y;
+    m.m[6] = ((REAL)2 * (yz + wx)) * scale.y;
+    m.m[7] = 0;
+    m.m[8] = ((REAL)2 * (xz + wy)) * scale.z;
+    m.m[9] = ((REAL)2 * (yz - wx)) * scale.z;
+    m.m[10] = ((REAL)1 - (REAL)2 * (xx + yy)) * scale.z;
+    m.m[11] = 0;
+    m.m[12] = pos.x;
+    m.m[13] = pos.y;
+    m.m[14] = pos.z;
+    m.m[15] = 1;
+    return m;
+}
+
+/* q <- normalize(q + dt/2 (0, W) q): W is the world-frame angular velocity in
+ * rad/s, the quantity the Constraint terms (W x N, code/nans.cpp:1149) treat it
+ * as.  In exact arithmetic one step rotates by 2 atan(|W| dt / 2) about W. */
+static void SUF(quat_step)(REAL *q, V3 W, REAL dt)
+{
+    const REAL h = (REAL)0.5 * dt;
+    const REAL w = q[0], x = q[1], y = q[2], z = q[3];
+    const REAL nw = w - h * ((W.x * x + W.y * y) + W.z * z);
+    const REAL nx = x + h * (w * W.x + (W.y * z - W.z * y));
+    const REAL ny = y + h * (w * W.y + (W.z * x - W.x * z));
+    const REAL nz = z + h * (w * W.z + (W.x * y - W.y * x));
+    const REAL inv = (REAL)1 / SQRT(((nw * nw + nx * nx) + ny * ny) + nz * nz);
+    q[0] = nw * inv;
+    q[1] = nx * inv;
+    q[2] = ny * inv;
+    q[3] = nz * inv;
+}
+
 /* ---------------------------------------------------------------- shapes, support */
 typedef struct {
     int kind;       /* ORA_SHAPE_BOX / ORA_SHAPE_SPHERE */
@@ -668,6 +716,7 @@ int SUF(ora_narrowphase)(int64_t n,
 /* ---------------------------------------------------------------- world step */
 typedef struct {
     V3 P, V, F, A, W, T, S;
+    REAL q[4];      /* orientation in quaternion mode (w, x, y, z) */
     REAL mass, moi;
     M4 model;
     V3 verts[8];
@@ -679,6 +728,7 @@ typedef struct {
     Body *box;      /* Cubes[] */
     Body *sph;      /* Spheres[] */
     Body floor;     /* Floor (a box, nans.cpp:1666-1678) */
+    int quat;       /* 1: bodies carry quaternions (opt-in), 0: Euler degrees */
 } SUF(World);
 #define World SUF(World)
 
@@ -689,9 +739,9 @@ typedef struct {
 } SUF(Contact);
 #define Contact SUF(Contact)
 
-static void SUF(rebuild_model)(Body *b)
+static void SUF(rebuild_model)(Body *b, int quat)
 {
-    b->model = SUF(model_from_pose)(b->P, b->A, b->S);
+    b->model = quat ? SUF(model_from_quat)(b->P, b->q, b->S) : SUF(model_from_pose)(b->P, b->A, b->S);
     SUF(update_vertices)(&b->model, b->verts);
 }
 
@@ -1026,29 +1076,33 @@ static void SUF(integrate_velocities)(World *w, const ora_step_params *p)
     for (int i = 0; i < w->nb; ++i) {
         Body *b = &w->box[i];
         b->P = vadd(b->P, vmul(b->V, dt));
-        b->A = vadd(b->A, vmul(b->W, dt));
+        if (w->quat) SUF(quat_step)(b->q, b->W, dt);
+        else b->A = vadd(b->A, vmul(b->W, dt));
         SUF(update_vertices)(&b->model, b->verts);
     }
     for (int i = 0; i < w->ns; ++i) {
         Body *b = &w->sph[i];
         b->P = vadd(b->P, vmul(b->V, dt));
-        b->A = vadd(b->A, vmul(b->W, dt));
+        if (w->quat) SUF(quat_step)(b->q, b->W, dt);
+        else b->A = vadd(b->A, vmul(b->W, dt));
     }
 }
 
 static void SUF(rebuild_models)(World *w)
 {
-    SUF(rebuild_model)(&w->floor);
+    SUF(rebuild_model)(&w->floor, 0);
     for (int i = 0; i < w->nb; ++i)
-        SUF(rebuild_model)(&w->box[i]);
+        SUF(rebuild_model)(&w->box[i], w->quat);
 }
 
 static void SUF(world_load)(World *w, int32_t nb, int32_t ns, const REAL *pos, const REAL *vel,
                             const REAL *angvel, const REAL *angles, const REAL *force, const REAL *torque,
-                            const REAL *scale, const REAL *mass, const REAL *moi, const ora_step_params *p)
+                            const REAL *scale, const REAL *mass, const REAL *moi, const ora_step_params *p,
+                            const REAL *quat)
 {
     w->nb = nb;
     w->ns = ns;
+    w->quat = quat != NULL;
     w->box = (Body *)calloc((size_t)(nb > 0 ? nb : 1), sizeof(Body));
     w->sph = (Body *)calloc((size_t)(ns > 0 ? ns : 1), sizeof(Body));
     for (int k = 0; k < nb + ns; ++k) {
@@ -1062,6 +1116,7 @@ static void SUF(world_load)(World *w, int32_t nb, int32_t ns, const REAL *pos, c
         b->S = mk3(scale[3 * k], scale[3 * k + 1], scale[3 * k + 2]);
         b->mass = mass[k];
         b->moi = moi[k];
+        for (int c = 0; c < 4; ++c) b->q[c] = quat ? quat[4 * k + c] : (REAL)(c == 0);
     }
     memset(&w->floor, 0, sizeof(w->floor));
     w->floor.P = mk3((REAL)p->floor_pos[0], (REAL)p->floor_pos[1], (REAL)p->floor_pos[2]);
@@ -1072,10 +1127,12 @@ static void SUF(world_load)(World *w, int32_t nb, int32_t ns, const REAL *pos, c
 }
 
 static void SUF(world_store)(const World *w, REAL *pos, REAL *vel, REAL *angvel, REAL *angles,
-                             REAL *force, REAL *torque)
+                             REAL *force, REAL *torque, REAL *quat)
 {
     for (int k = 0; k < w->nb + w->ns; ++k) {
         const Body *b = k < w->nb ? &w->box[k] : &w->sph[k - w->nb];
+        if (quat)
+            for (int c = 0; c < 4; ++c) quat[4 * k + c] = b->q[c];
         pos[3 * k] = b->P.x; pos[3 * k + 1] = b->P.y; pos[3 * k + 2] = b->P.z;
         vel[3 * k] = b->V.x; vel[3 * k + 1] = b->V.y; vel[3 * k + 2] = b->V.z;
         angvel[3 * k] = b->W.x; angvel[3 * k + 1] = b->W.y; angvel[3 * k + 2] = b->W.z;
@@ -1096,19 +1153,19 @@ static int SUF(params_ok)(const ora_step_params *p)
 #define ORA_CONTACT_T ora_contact_f64
 #endif
 
-int SUF(ora_world_run)(int32_t nb, int32_t ns,
-                       REAL *pos, REAL *vel, REAL *angvel, REAL *angles,
-                       REAL *force, REAL *torque,
-                       const REAL *scale, const REAL *mass, const REAL *moi,
-                       const ora_step_params *p, int32_t nsteps,
-                       ORA_CONTACT_T *contacts, int32_t max_contacts, int32_t *ncontacts,
-                       int64_t *stats)
+static int SUF(world_run)(int32_t nb, int32_t ns,
+                          REAL *pos, REAL *vel, REAL *angvel, REAL *angles, REAL *quat,
+                          REAL *force, REAL *torque,
+                          const REAL *scale, const REAL *mass, const REAL *moi,
+                          const ora_step_params *p, int32_t nsteps,
+                          ORA_CONTACT_T *contacts, int32_t max_contacts, int32_t *ncontacts,
+                          int64_t *stats)
 {
     if (nb < 0 || ns < 0 || nsteps < 0 || !SUF(params_ok)(p) || !pos || !vel || !angvel || !angles ||
         !scale || !mass || !moi)
         return -1;
     World w;
-    SUF(world_load)(&w, nb, ns, pos, vel, angvel, angles, force, torque, scale, mass, moi, p);
+    SUF(world_load)(&w, nb, ns, pos, vel, angvel, angles, force, torque, scale, mass, moi, p, quat);
     ContactList cl;
     memset(&cl, 0, sizeof(cl));
     int64_t st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1152,11 +1209,36 @@ int SUF(ora_world_run)(int32_t nb, int32_t ns,
     } else if (ncontacts) {
         *ncontacts = cl.n;
     }
-    SUF(world_store)(&w, pos, vel, angvel, angles, force, torque);
+    SUF(world_store)(&w, pos, vel, angvel, angles, force, torque, quat);
     free(cl.c);
     free(w.box);
     free(w.sph);
     return 0;
+}
+
+int SUF(ora_world_run)(int32_t nb, int32_t ns,
+                       REAL *pos, REAL *vel, REAL *angvel, REAL *angles,
+                       REAL *force, REAL *torque,
+                       const REAL *scale, const REAL *mass, const REAL *moi,
+                       const ora_step_params *p, int32_t nsteps,
+                       ORA_CONTACT_T *contacts, int32_t max_contacts, int32_t *ncontacts,
+                       int64_t *stats)
+{
+    return SUF(world_run)(nb, ns, pos, vel, angvel, angles, NULL, force, torque, scale, mass, moi, p, nsteps,
+                          contacts, max_contacts, ncontacts, stats);
+}
+
+int SUF(ora_world_run_quat)(int32_t nb, int32_t ns,
+                            REAL *pos, REAL *vel, REAL *angvel, REAL *angles, REAL *quat,
+                            REAL *force, REAL *torque,
+                            const REAL *scale, const REAL *mass, const REAL *moi,
+                            const ora_step_params *p, int32_t nsteps,
+                            ORA_CONTACT_T *contacts, int32_t max_contacts, int32_t *ncontacts,
+                            int64_t *stats)
+{
+    if (!quat) return -1;
+    return SUF(world_run)(nb, ns, pos, vel, angvel, angles, quat, force, torque, scale, mass, moi, p, nsteps,
+                          contacts, max_contacts, ncontacts, stats);
 }
 
 /* ---------------------------------------------------------------- frozen-manifold rollouts */
@@ -1178,7 +1260,7 @@ int SUF(ora_frozen_record)(int32_t nb, int32_t ns,
         !scale || !mass || !moi || !step_count || max_out < 0 || (max_out > 0 && !out))
         return -1;
     World w;
-    SUF(world_load)(&w, nb, ns, pos, vel, angvel, angles, force, torque, scale, mass, moi, p);
+    SUF(world_load)(&w, nb, ns, pos, vel, angvel, angles, force, torque, scale, mass, moi, p, NULL);
     ContactList cl;
     memset(&cl, 0, sizeof(cl));
     int64_t k = 0;
@@ -1213,7 +1295,7 @@ int SUF(ora_frozen_record)(int32_t nb, int32_t ns,
         SUF(rebuild_models)(&w);
     }
     if (total) *total = k;
-    SUF(world_store)(&w, pos, vel, angvel, angles, force, torque);
+    SUF(world_store)(&w, pos, vel, angvel, angles, force, torque, NULL);
     free(cl.c);
     free(w.box);
     free(w.sph);
@@ -1306,7 +1388,7 @@ int SUF(ora_world_pairs)(int32_t nb, int32_t ns,
     for (int k = 0; k < n; ++k)
         ones[k] = 1;
     World w;
-    SUF(world_load)(&w, nb, ns, pos, zero, zero, angles, NULL, NULL, scale, ones, ones, p);
+    SUF(world_load)(&w, nb, ns, pos, zero, zero, angles, NULL, NULL, scale, ones, ones, p, NULL);
     ContactList cl;
     memset(&cl, 0, sizeof(cl));
     int64_t k = 0;
@@ -1340,6 +1422,18 @@ int SUF(ora_world_pairs)(int32_t nb, int32_t ns,
     return 0;
 }
 #endif
+
+void SUF(ora_box_vertices_quat)(const REAL *pos, const REAL *quat, const REAL *scale, REAL *out)
+{
+    M4 m = SUF(model_from_quat)(mk3(pos[0], pos[1], pos[2]), quat, mk3(scale[0], scale[1], scale[2]));
+    V3 v[8];
+    SUF(update_vertices)(&m, v);
+    for (int k = 0; k < 8; ++k) {
+        out[3 * k] = v[k].x;
+        out[3 * k + 1] = v[k].y;
+        out[3 * k + 2] = v[k].z;
+    }
+}
 
 void SUF(ora_box_vertices)(const REAL *pos, const REAL *angles, const REAL *scale, REAL *out)
 {

@@ -138,6 +138,7 @@ class WorldState:
         self.force, self.torque = cp(scene.force), cp(scene.torque)
         self.scale, self.mass, self.moi = cp(scene.scale), cp(scene.mass), cp(scene.moi)
         self.params = scene.params
+        self.quat = None  # [n][4] (w, x, y, z) in the opt-in quaternion mode
         self.friction = None if scene.world_friction is None else float(scene.world_friction[w])
         self.restitution = None if scene.world_restitution is None else float(scene.world_restitution[w])
         self.stats = np.zeros(8, np.int64)
@@ -145,10 +146,15 @@ class WorldState:
     def run(self, nsteps: int, contacts: bool = False, max_contacts: int = 1 << 16, opt: str = "O2"):
         """Advance nsteps; returns the contact list of the last step if asked."""
         prm = step_params(self.params, self.friction, self.restitution)
-        fn = getattr(lib(opt), "ora_world_run_" + self.precision)
         cbuf = np.zeros(max_contacts, CONTACT_DTYPE[self.precision]) if contacts else None
         ncon = ctypes.c_int32(0)
-        rc = fn(self.nb, self.ns, _ptr(self.pos), _ptr(self.vel), _ptr(self.angvel), _ptr(self.angles),
+        state = [_ptr(self.pos), _ptr(self.vel), _ptr(self.angvel), _ptr(self.angles)]
+        if self.quat is None:
+            fn = getattr(lib(opt), "ora_world_run_" + self.precision)
+        else:
+            fn = getattr(lib(opt), "ora_world_run_quat_" + self.precision)
+            state.append(_ptr(self.quat))
+        rc = fn(self.nb, self.ns, *state,
                 _ptr(self.force), _ptr(self.torque), _ptr(self.scale), _ptr(self.mass), _ptr(self.moi),
                 ctypes.byref(prm), int(nsteps), _ptr(cbuf), int(max_contacts), ctypes.byref(ncon),
                 _ptr(self.stats))
@@ -171,6 +177,15 @@ class WorldState:
         if rc != 0:
             raise ValueError("oracle world_pairs rejected its arguments")
         return t, ia, ib, st
+
+
+def box_vertices_quat(pos, quat, scale, precision: str = "f32"):
+    """Corners of the quaternion-mode Model [R(q) diag(scale) | pos], [8][3]."""
+    ft = np.float32 if precision == "f32" else np.float64
+    out = np.zeros((8, 3), ft)
+    a = [np.ascontiguousarray(x, dtype=ft) for x in (pos, quat, scale)]
+    getattr(lib(), "ora_box_vertices_quat_" + precision)(*[_ptr(x) for x in a], _ptr(out))
+    return out
 
 
 def box_vertices(pos, angles, scale, precision: str = "f32"):
