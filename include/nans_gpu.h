@@ -228,6 +228,19 @@ int nans_set_orientation_mode(nans_ctx *ctx, int32_t mode);
 int nans_set_quaternions(nans_ctx *ctx, const float *quat);
 int nans_get_quaternions(nans_ctx *ctx, float *quat);
 
+/* Warm start of the plain step (opt-in; no counterpart in the reference, whose
+ * DetectCollisions marks "Warm starting?" as a TODO where a contact pair of the
+ * previous frame is found again, code/nans.cpp:1374-1393).  When enabled, each
+ * contact whose pair (same kind and indices) was a contact in the previous step
+ * starts its accumulators DLNormalSum, DLTangent1Sum, DLTangent2Sum from that
+ * step's final values instead of zero; everything else, including the applied
+ * impulse being the last iteration's delta (code/nans.cpp:1176-1238), is
+ * unchanged.  At most 16 n + 64 contacts of a world carry over; the others start
+ * from zero.  Enabling it, nans_set_state and nans_upload_worlds empty the
+ * lists.  Results differ from the reference's by design.  The batched step, its
+ * autotuner and the differentiable rollout refuse the mode. */
+int nans_set_warm_start(nans_ctx *ctx, int32_t enable);
+
 /* Broadphase of the plain step.  The reference tests every pair of a world
  * (code/nans.cpp:1355-1535); the plain step keeps the pairs whose bounding
  * spheres can touch, found either by testing every canonical pair or with a

@@ -328,6 +328,7 @@ int nans_rollout(nans_ctx *ctx, int32_t nsteps, int32_t checkpoint_every)
     NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
     NANS_REQUIRE(ctx->orientation == NANS_ORIENT_EULER, NANS_ERR_STATE,
                  "the differentiable rollout follows the reference's Euler orientation only");
+    NANS_REQUIRE(!ctx->warm, NANS_ERR_STATE, "the differentiable rollout does not warm-start contacts");
     NANS_REQUIRE(nsteps >= 1 && checkpoint_every >= 1, NANS_ERR_INVALID,
                  "nsteps and checkpoint_every must be >= 1 (got %d, %d)", nsteps, checkpoint_every);
     NANS_REQUIRE(ctx->p.solver_iterations <= ADJ_MAX_ITERS, NANS_ERR_INVALID,

@@ -74,6 +74,7 @@ inline int dmalloc(T **p, size_t n)
 
 void wfree(void *p);
 StepConsts step_consts(const nans_params &p, int orientation);
+int warm_reset(nans_ctx *ctx);
 int setup_floor(nans_ctx *ctx, hipStream_t s);
 int check_overflow(nans_ctx *ctx);
 // nsteps batched steps replayed from the captured one-step graph
@@ -129,6 +130,15 @@ struct nans_worlds {
     int *bp_counts = nullptr;    // [3 nmax] row counts
     int *bp_offsets = nullptr;   // [3 nmax]
     int *bp_tmp = nullptr;       // [cand_cap] unsorted rows
+    // opt-in warm start of the plain step: two contact lists per world
+    int warm_ready = 0, warm_flip = 0;
+    std::vector<int64_t> h_warm_off;
+    std::vector<int> h_warm_cap;
+    int *warm_p[2] = {nullptr, nullptr};    // canonical pair indices
+    float *warm_s[2] = {nullptr, nullptr};  // [3] accumulators per contact
+    int *warm_n[2] = {nullptr, nullptr};    // [nworlds] list lengths
+    int *warm_cp = nullptr;                 // [cand_cap] pair index per contact of the world being solved
+    float *warm_sums = nullptr;             // [3 cand_cap] its accumulators
     void *bp_grid = nullptr;     // nans::BpGrid
     // batched scratch
     int b_grid = 0;

@@ -144,10 +144,10 @@ constexpr int SOLVE_REG = 2;
 // pre[0..npre) (may be empty) receives Constraint's position-only terms of the
 // first npre contacts in one parallel pass, so that the rounds only evaluate the
 // velocity-dependent part for them.
-template <int BLOCK>
+template <int BLOCK, bool WARM = false>
 __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int ncon, int *done, int *head,
                                     const FloorDev &fl, const StepConsts &k, float Cf, float Cr,
-                                    ContactTerms *pre = nullptr, int npre = 0)
+                                    ContactTerms *pre = nullptr, int npre = 0, float *warm = nullptr)
 {
     const int tid = threadIdx.x;
     npre = min(npre, ncon);
@@ -186,8 +186,9 @@ __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int
         int left = 0;
         auto apply = [&](int c) {
             ContactRec r = con[c];
-            if (c < npre) contact_apply(B, r, pre[c], k, Cf, Cr);
-            else constraint(B, r, fl, k, Cf, Cr);
+            float *sums = WARM ? warm + 3 * c : nullptr;
+            if (c < npre) contact_apply(B, r, pre[c], k, Cf, Cr, sums);
+            else constraint(B, r, fl, k, Cf, Cr, sums);
             con[c].dln = r.dln;
             con[c].dlt1 = r.dlt1;
             con[c].dlt2 = r.dlt2;
@@ -371,10 +372,23 @@ struct SolveArgs {
     int rec_cap;
     int *rec_count;
     unsigned long long *cnt;
+    // opt-in warm start: the world's contact lists of the previous and this step
+    // (canonical pair index, three accumulators; ascending), scratch per contact
+    const int *cand;
+    const int *warm_p_prev;
+    const float *warm_s_prev;
+    const int *warm_n_prev;
+    int *warm_p_next;
+    float *warm_s_next;
+    int *warm_n_next;
+    int warm_cap;
+    int *warm_cp;
+    float *warm_sums;
 };
 
 // Ordered contact compaction, dependency-round solve and IntegrateVelocities of
 // one world, by one workgroup; head[] in dynamic LDS.
+template <bool WARM>
 __global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
 {
     extern __shared__ int head[];
@@ -389,7 +403,10 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
         bool hit = c < total && a.flag[c];
         int tot;
         int r = block_rank<SOLVE_THREADS>(hit, wave_tot, tot);
-        if (hit && ncon + r < a.con_cap) a.con[ncon + r] = a.rec[c];
+        if (hit && ncon + r < a.con_cap) {
+            a.con[ncon + r] = a.rec[c];
+            if (WARM) a.warm_cp[ncon + r] = a.cand[c];
+        }
         ncon += tot;
     }
     if (ncon > a.con_cap) {
@@ -398,7 +415,33 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
     }
     if (threadIdx.x == 0) atomicAdd(&a.cnt[CNT_CONTACTS], (unsigned long long)ncon);
     __syncthreads();
-    solve_rounds<SOLVE_THREADS>(a.B, a.n, a.con, ncon, a.done, head, fl, a.k, a.Cf, a.Cr);
+    if (WARM) {
+        // accumulators of the same pair in the previous step's list, else zero
+        const int np = *a.warm_n_prev;
+        for (int c = threadIdx.x; c < ncon; c += SOLVE_THREADS) {
+            const int key = a.warm_cp[c];
+            int lo = 0, hi = np;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (a.warm_p_prev[mid] < key) lo = mid + 1;
+                else hi = mid;
+            }
+            const bool hit = lo < np && a.warm_p_prev[lo] == key;
+            for (int q = 0; q < 3; ++q) a.warm_sums[3 * c + q] = hit ? a.warm_s_prev[3 * lo + q] : 0.0f;
+        }
+        __syncthreads();
+    }
+    solve_rounds<SOLVE_THREADS, WARM>(a.B, a.n, a.con, ncon, a.done, head, fl, a.k, a.Cf, a.Cr, nullptr, 0,
+                                      a.warm_sums);
+    if (WARM) {
+        __syncthreads();
+        const int m = min(ncon, a.warm_cap);
+        for (int c = threadIdx.x; c < m; c += SOLVE_THREADS) {
+            a.warm_p_next[c] = a.warm_cp[c];
+            for (int q = 0; q < 3; ++q) a.warm_s_next[3 * c + q] = a.warm_sums[3 * c + q];
+        }
+        if (threadIdx.x == 0) *a.warm_n_next = m;
+    }
     for (int i = threadIdx.x; i < a.n; i += SOLVE_THREADS) {
         if (a.k.quat) integrate_velocities_quat(a.B, a.B, i, a.k.dt);
         else integrate_velocities_body(a.B, i, a.k.dt);
@@ -628,7 +671,9 @@ int worlds_release(nans_ctx *ctx)
                     (void *)W->blk_counts, (void *)W->blk_offsets, (void *)W->ncand, (void *)W->queue,
                     (void *)W->bp_cell, (void *)W->bp_hslot, (void *)W->bp_sorted,
                     (void *)W->bp_bcnt, (void *)W->bp_boff, (void *)W->bp_counts, (void *)W->bp_offsets,
-                    (void *)W->bp_tmp, W->bp_grid})
+                    (void *)W->bp_tmp, W->bp_grid, (void *)W->warm_p[0], (void *)W->warm_p[1], (void *)W->warm_s[0],
+                    (void *)W->warm_s[1], (void *)W->warm_n[0], (void *)W->warm_n[1], (void *)W->warm_cp,
+                    (void *)W->warm_sums})
         wfree(p);
     for (hipEvent_t ev : {W->ev_in, W->ev_out, W->ev_t0, W->ev_t1})
         if (ev) hipEventDestroy(ev);
@@ -736,6 +781,42 @@ static bool use_grid(const nans_ctx *ctx, int n)
     if (ctx->broadphase == NANS_BROADPHASE_GRID) return true;
     if (ctx->broadphase == NANS_BROADPHASE_ALL_PAIRS) return false;
     return n >= GRID_MIN_BODIES;
+}
+
+// Contact lists of the opt-in warm start, allocated on its first step: per
+// world, two lists (the previous step's and this step's) of at most 16 n + 64
+// contacts, and per-contact scratch of the size of the candidate scratch.
+static int warm_prepare(nans_ctx *ctx)
+{
+    nans_worlds *W = ctx->worlds;
+    if (W->warm_ready) return NANS_OK;
+    W->h_warm_off.assign(W->nworlds + 1, 0);
+    W->h_warm_cap.assign(W->nworlds, 0);
+    for (int w = 0; w < W->nworlds; ++w) {
+        W->h_warm_cap[w] = 16 * (W->h_off[w + 1] - W->h_off[w]) + 64;
+        W->h_warm_off[w + 1] = W->h_warm_off[w] + W->h_warm_cap[w];
+    }
+    const size_t total = (size_t)W->h_warm_off[W->nworlds];
+    for (int b = 0; b < 2; ++b) {
+        NANS_TRY(dmalloc(&W->warm_p[b], total));
+        NANS_TRY(dmalloc(&W->warm_s[b], 3 * total));
+        NANS_TRY(dmalloc(&W->warm_n[b], (size_t)W->nworlds));
+    }
+    NANS_TRY(dmalloc(&W->warm_cp, (size_t)W->cand_cap));
+    NANS_TRY(dmalloc(&W->warm_sums, (size_t)3 * W->cand_cap));
+    W->warm_ready = 1;
+    return warm_reset(ctx);
+}
+
+// Empties the warm-start lists (the next step starts every contact from zero).
+int warm_reset(nans_ctx *ctx)
+{
+    nans_worlds *W = ctx->worlds;
+    if (!W || !W->warm_ready) return NANS_OK;
+    for (int b = 0; b < 2; ++b) k_zero_i32<<<(W->nworlds + 255) / 256, 256, 0, ctx->stream>>>(W->warm_n[b], W->nworlds);
+    NANS_HIP(hipGetLastError());
+    W->warm_flip = 0;
+    return NANS_OK;
 }
 
 // Candidates of one world from the hierarchical grid (nans_broadphase.hpp), in
@@ -894,8 +975,9 @@ int nans_upload_worlds(nans_ctx *ctx, int32_t nworlds, const int32_t *world_offs
     NANS_TRY(dmalloc(&W->bp_offsets, (size_t)3 * nmax));
     NANS_TRY(dmalloc(&W->bp_tmp, (size_t)cap));
     NANS_TRY(dmalloc((BpGrid **)&W->bp_grid, 1));
-    NANS_HIP(hipFuncSetAttribute((const void *)k_plain_solve, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(sizeof(int) * PLAIN_MAX_BODIES)));
+    for (const void *f : {(const void *)k_plain_solve<false>, (const void *)k_plain_solve<true>})
+        NANS_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(sizeof(int) * PLAIN_MAX_BODIES)));
     k_zero_u64<<<1, 64>>>(W->cnt, CNT_N);
     NANS_HIP(hipGetLastError());
     NANS_TRY(setup_floor(ctx, nullptr));
@@ -1017,6 +1099,7 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
     nans_worlds *W = ctx->worlds;
     NANS_HIP(hipSetDevice(ctx->p.device));
     hipStream_t s = ctx->stream;
+    if (ctx->warm) NANS_TRY(warm_prepare(ctx));
     NANS_TRY(setup_floor(ctx, s));
     const StepConsts k = step_consts(ctx->p, ctx->orientation);
     for (int step = 0; step < nsteps; ++step) {
@@ -1062,9 +1145,26 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
             a.rec_cap = W->record ? W->h_rec_cap[w] : 0;
             a.rec_count = W->record ? W->rec_count + w : nullptr;
             a.cnt = W->cnt;
-            k_plain_solve<<<1, SOLVE_THREADS, sizeof(int) * n, s>>>(a);
+            a.cand = W->cand;
+            if (ctx->warm) {
+                const int cur = W->warm_flip, nxt = cur ^ 1;
+                const int64_t off = W->h_warm_off[w];
+                a.warm_p_prev = W->warm_p[cur] + off;
+                a.warm_s_prev = W->warm_s[cur] + 3 * off;
+                a.warm_n_prev = W->warm_n[cur] + w;
+                a.warm_p_next = W->warm_p[nxt] + off;
+                a.warm_s_next = W->warm_s[nxt] + 3 * off;
+                a.warm_n_next = W->warm_n[nxt] + w;
+                a.warm_cap = W->h_warm_cap[w];
+                a.warm_cp = W->warm_cp;
+                a.warm_sums = W->warm_sums;
+                k_plain_solve<true><<<1, SOLVE_THREADS, sizeof(int) * n, s>>>(a);
+            } else {
+                k_plain_solve<false><<<1, SOLVE_THREADS, sizeof(int) * n, s>>>(a);
+            }
             NANS_HIP(hipGetLastError());
         }
+        if (ctx->warm) W->warm_flip ^= 1;
     }
     NANS_HIP(hipStreamSynchronize(s));
     return check_overflow(ctx);
@@ -1251,6 +1351,7 @@ extern "C" {
 int nans_step_batched(nans_ctx *ctx, int32_t nsteps)
 {
     NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
+    NANS_REQUIRE(!ctx->warm, NANS_ERR_STATE, "warm starting is a mode of the plain step");
     NANS_REQUIRE(nsteps >= 0, NANS_ERR_INVALID, "nsteps < 0");
     if (nsteps == 0) return NANS_OK;
     NANS_HIP(hipSetDevice(ctx->p.device));
@@ -1291,6 +1392,7 @@ int nans_set_batched_block(nans_ctx *ctx, int32_t threads_per_world)
 int nans_autotune_batched(nans_ctx *ctx, int32_t nsteps, nans_tuning *out)
 {
     NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
+    NANS_REQUIRE(!ctx->warm, NANS_ERR_STATE, "warm starting is a mode of the plain step");
     NANS_REQUIRE(nsteps >= 1, NANS_ERR_INVALID, "nsteps must be >= 1");
     nans_worlds *W = ctx->worlds;
     NANS_REQUIRE(W->nmax <= BATCH_MAX_BODIES, NANS_ERR_CAPACITY,
@@ -1412,6 +1514,15 @@ int nans_set_state(nans_ctx *ctx, const nans_bodies_soa *b)
     if (b->force) std::copy(b->force, b->force + 3 * N, ft.begin());
     if (b->torque) std::copy(b->torque, b->torque + 3 * N, ft.begin() + 3 * N);
     NANS_HIP(hipMemcpy(W->planes + (size_t)F_FX * N, ft.data(), sizeof(float) * ft.size(), hipMemcpyHostToDevice));
+    return warm_reset(ctx);
+}
+
+int nans_set_warm_start(nans_ctx *ctx, int32_t enable)
+{
+    NANS_REQUIRE(ctx, NANS_ERR_INVALID, "ctx is NULL");
+    NANS_REQUIRE(enable == 0 || enable == 1, NANS_ERR_INVALID, "enable must be 0 or 1");
+    if (enable && !ctx->warm) NANS_TRY(warm_reset(ctx));
+    ctx->warm = enable;
     return NANS_OK;
 }
 

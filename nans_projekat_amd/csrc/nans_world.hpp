@@ -376,8 +376,11 @@ __host__ __device__ inline void contact_terms(const Planes &B, const ContactRec 
 // The rest of Constraint (code/nans.cpp:1148-1329): relative velocities from the
 // current V, W of the bodies, the impulse iterations, and the application.
 // friction / restitution are the pair's coefficients.
+// sums (may be null): the accumulators DLNormalSum, DLTangent1Sum, DLTangent2Sum
+// at entry (null: zero, as every contact_pair starts) and at exit, for the
+// opt-in warm start.
 __host__ __device__ inline void contact_apply(const Planes &B, ContactRec &c, const ContactTerms &t,
-                                              const StepConsts &k, float Cf, float Cr)
+                                              const StepConsts &k, float Cf, float Cr, float *sums = nullptr)
 {
     const V3 zero = mk3(0.0f, 0.0f, 0.0f);
     const V3 V1 = B.get3(F_VX, c.slot_a), W1 = B.get3(F_WX, c.slot_a);
@@ -393,6 +396,11 @@ __host__ __device__ inline void contact_apply(const Planes &B, ContactRec &c, co
     float JdVt2 = dot(dVt2, T2);
 
     float sumN = 0.0f, sumT1 = 0.0f, sumT2 = 0.0f, dln = 0.0f, dlt1 = 0.0f, dlt2 = 0.0f;
+    if (sums) {
+        sumN = sums[0];
+        sumT1 = sums[1];
+        sumT2 = sums[2];
+    }
     const double cf_sqrt2 = 1.4142135623730951 * (double)Cf;
     for (int it = 0; it < k.solver_iterations; ++it) {
         float LambdaN = (-JdVn + Bb) * t.JMJn;
@@ -417,6 +425,11 @@ __host__ __device__ inline void contact_apply(const Planes &B, ContactRec &c, co
     c.dln = dln;
     c.dlt1 = dlt1;
     c.dlt2 = dlt2;
+    if (sums) {
+        sums[0] = sumN;
+        sums[1] = sumT1;
+        sums[2] = sumT2;
+    }
     const float InvM1 = t.InvM1, InvM2 = t.InvM2, InvI1 = t.InvI1, InvI2 = t.InvI2;
     V3 LinN = mul(N, dln), LinT1 = mul(T1, dlt1), LinT2 = mul(T2, dlt2);
     V3 AngI1 = mul(t.RN1, dln), AngI2 = mul(t.RN2, dln);
@@ -446,11 +459,11 @@ __host__ __device__ inline void contact_apply(const Planes &B, ContactRec &c, co
 // Solve one contact against the current V, W of its bodies and apply the impulses
 // (code/nans.cpp:1020-1329).
 __host__ __device__ inline void constraint(const Planes &B, ContactRec &c, const FloorDev &fl, const StepConsts &k,
-                                           float Cf, float Cr)
+                                           float Cf, float Cr, float *sums = nullptr)
 {
     ContactTerms t;
     contact_terms(B, c, fl, k, t);
-    contact_apply(B, c, t, k, Cf, Cr);
+    contact_apply(B, c, t, k, Cf, Cr, sums);
 }
 
 }  // namespace nans

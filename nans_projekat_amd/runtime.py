@@ -296,6 +296,10 @@ class Context:
     def set_batched_block(self, threads_per_world: int) -> None:
         check(self.lib.nans_set_batched_block(self.h, int(threads_per_world)))
 
+    def set_warm_start(self, enable: bool) -> None:
+        """Opt-in warm start of the plain step (contact accumulators carried over)."""
+        check(self.lib.nans_set_warm_start(self.h, 1 if enable else 0))
+
     ORIENTATION = {"euler": 0, "quaternion": 1}
 
     def set_orientation_mode(self, mode: str) -> None:

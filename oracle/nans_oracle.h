@@ -178,6 +178,31 @@ int ora_world_run_quat_f64(int32_t nb, int32_t ns,
                            ora_contact_f64 *contacts, int32_t max_contacts, int32_t *ncontacts,
                            int64_t *stats);
 
+/* ora_world_run with the opt-in warm start (no counterpart in the reference; its
+ * DetectCollisions marks "Warm starting?" as a TODO on a cache hit,
+ * code/nans.cpp:1374-1393).  warm_pair / warm_sums / *warm_n (in/out, capacity
+ * warm_cap) list the previous step's contacts as canonical pair indices
+ * (ascending) with their final accumulators DLNormalSum, DLTangent1Sum,
+ * DLTangent2Sum; a contact of the same pair starts from those accumulators
+ * instead of zero, everything else as ora_world_run (or, with quat != NULL, as
+ * ora_world_run_quat).  After each step the list holds that step's first
+ * warm_cap contacts. */
+int ora_world_run_warm_f32(int32_t nb, int32_t ns,
+                           float *pos, float *vel, float *angvel, float *angles, float *quat,
+                           float *force, float *torque,
+                           const float *scale, const float *mass, const float *moi,
+                           const ora_step_params *p, int32_t nsteps,
+                           ora_contact_f32 *contacts, int32_t max_contacts, int32_t *ncontacts,
+                           int64_t *stats, int64_t *warm_pair, float *warm_sums, int32_t *warm_n, int32_t warm_cap);
+int ora_world_run_warm_f64(int32_t nb, int32_t ns,
+                           double *pos, double *vel, double *angvel, double *angles, double *quat,
+                           double *force, double *torque,
+                           const double *scale, const double *mass, const double *moi,
+                           const ora_step_params *p, int32_t nsteps,
+                           ora_contact_f64 *contacts, int32_t max_contacts, int32_t *ncontacts,
+                           int64_t *stats, int64_t *warm_pair, double *warm_sums, int32_t *warm_n,
+                           int32_t warm_cap);
+
 /* Every canonical pair of the world's current pose (DetectCollisions order,
  * nans.cpp:1355-1535): type/index_a/index_b and the per-pair stats of
  * CheckCollision, without solving or stepping.  Arrays have room for

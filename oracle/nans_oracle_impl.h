@@ -950,9 +950,12 @@ static void SUF(contact_frame)(World *w, const Contact *pair, V3 *N, V3 *R1, V3 
  * B's velocities are updated (B's only when two_sided).  dl[3] receives the
  * applied impulses. */
 static void SUF(constraint_solve)(Body *A, Body *B, int two_sided, V3 N, V3 R1, V3 R2,
-                                  const ora_step_params *p, REAL Cf, REAL Cr, REAL *dl);
+                                  const ora_step_params *p, REAL Cf, REAL Cr, REAL *dl, REAL *sums);
 
-static void SUF(constraint)(World *w, Contact *pair, const ora_step_params *p)
+/* sums (may be NULL): the accumulators DLNormalSum, DLTangent1Sum, DLTangent2Sum
+ * at entry (NULL: zero, as every contact_pair starts, nans.cpp:1365) and at exit;
+ * the opt-in warm start carries them from one step to the next. */
+static void SUF(constraint)(World *w, Contact *pair, const ora_step_params *p, REAL *sums)
 {
     Body *A, *B;
     int ka, kb;
@@ -961,14 +964,14 @@ static void SUF(constraint)(World *w, Contact *pair, const ora_step_params *p)
     SUF(contact_frame)(w, pair, &N, &R1, &R2);
     int two_sided = pair->type == ORA_CC || pair->type == ORA_CS || pair->type == ORA_SS;
     REAL dl[3];
-    SUF(constraint_solve)(A, B, two_sided, N, R1, R2, p, (REAL)p->friction, (REAL)p->restitution, dl);
+    SUF(constraint_solve)(A, B, two_sided, N, R1, R2, p, (REAL)p->friction, (REAL)p->restitution, dl, sums);
     pair->dln = dl[0];
     pair->dlt1 = dl[1];
     pair->dlt2 = dl[2];
 }
 
 static void SUF(constraint_solve)(Body *A, Body *B, int two_sided, V3 N, V3 R1, V3 R2,
-                                  const ora_step_params *p, REAL Cf, REAL Cr, REAL *dl)
+                                  const ora_step_params *p, REAL Cf, REAL Cr, REAL *dl, REAL *sums)
 {
     V3 PosA = A->P, PosB = B->P;
     REAL InvI1 = (REAL)1 / A->moi, InvI2 = (REAL)1 / B->moi;
@@ -1007,6 +1010,11 @@ static void SUF(constraint_solve)(Body *A, Body *B, int two_sided, V3 N, V3 R1, 
     REAL JdVt2 = vdot(dVt2, T2);
 
     REAL sumN = 0, sumT1 = 0, sumT2 = 0, dln = 0, dlt1 = 0, dlt2 = 0;
+    if (sums) {
+        sumN = sums[0];
+        sumT1 = sums[1];
+        sumT2 = sums[2];
+    }
     int iter = p->solver_iterations;
     OC(OC_CONSTRAINT, 1);
     OC(OC_SOLVER_IT, iter);
@@ -1045,6 +1053,11 @@ static void SUF(constraint_solve)(Body *A, Body *B, int two_sided, V3 N, V3 R1, 
     dl[0] = dln;
     dl[1] = dlt1;
     dl[2] = dlt2;
+    if (sums) {
+        sums[0] = sumN;
+        sums[1] = sumT1;
+        sums[2] = sumT2;
+    }
     V3 LinearImpulse = vmul(N, dln);
     V3 LinearImpulseT1 = vmul(T1, dlt1);
     V3 LinearImpulseT2 = vmul(T2, dlt2);
@@ -1153,13 +1166,39 @@ static int SUF(params_ok)(const ora_step_params *p)
 #define ORA_CONTACT_T ora_contact_f64
 #endif
 
+/* Canonical index of a pair (DetectCollisions order, nans.cpp:1355-1535). */
+static int64_t SUF(pair_index)(int type, int ia, int ib, int64_t nb, int64_t ns)
+{
+    const int64_t ncc = nb * (nb - 1) / 2;
+    switch (type) {
+    case ORA_CC: return (int64_t)ia * (2 * nb - ia - 1) / 2 + (ib - ia - 1);
+    case ORA_CF: return ncc + ia;
+    case ORA_SF: return ncc + nb + ia;
+    case ORA_CS: return ncc + nb + ns + (int64_t)ia * ns + ib;
+    default: return ncc + nb + ns + nb * ns + (int64_t)ia * (2 * ns - ia - 1) / 2 + (ib - ia - 1);
+    }
+}
+
+/* Opt-in warm start (no counterpart in the reference, whose DetectCollisions
+ * leaves "Warm starting?" as a TODO on a cache hit, nans.cpp:1374-1393): the
+ * previous step's contacts as (canonical pair index, three accumulators), sorted
+ * by index, at most cap of them; a contact whose pair is listed starts from those
+ * accumulators. */
+typedef struct {
+    int64_t *p;
+    REAL *sums;
+    int32_t *n;
+    int32_t cap;
+} SUF(Warm);
+#define Warm SUF(Warm)
+
 static int SUF(world_run)(int32_t nb, int32_t ns,
                           REAL *pos, REAL *vel, REAL *angvel, REAL *angles, REAL *quat,
                           REAL *force, REAL *torque,
                           const REAL *scale, const REAL *mass, const REAL *moi,
                           const ora_step_params *p, int32_t nsteps,
                           ORA_CONTACT_T *contacts, int32_t max_contacts, int32_t *ncontacts,
-                          int64_t *stats)
+                          int64_t *stats, const Warm *warm)
 {
     if (nb < 0 || ns < 0 || nsteps < 0 || !SUF(params_ok)(p) || !pos || !vel || !angvel || !angles ||
         !scale || !mass || !moi)
@@ -1174,8 +1213,33 @@ static int SUF(world_run)(int32_t nb, int32_t ns,
         SUF(integrate_forces)(&w, p);
         SUF(detect_collisions)(&w, &cl, p);
         /* SolveConstraints (nans.cpp:1538-1548): in list order, on copies */
-        for (int k = 0; k < cl.n; ++k)
-            SUF(constraint)(&w, &cl.c[k], p);
+        if (!warm) {
+            for (int k = 0; k < cl.n; ++k)
+                SUF(constraint)(&w, &cl.c[k], p, NULL);
+        } else {
+            const int m = cl.n < warm->cap ? cl.n : warm->cap;
+            int64_t *np_ = (int64_t *)malloc(sizeof(int64_t) * (size_t)(m > 0 ? m : 1));
+            REAL *ns_ = (REAL *)malloc(sizeof(REAL) * 3 * (size_t)(m > 0 ? m : 1));
+            int j = 0;  /* both lists ascend: one merge pass finds the matches */
+            for (int k = 0; k < cl.n; ++k) {
+                const Contact *c = &cl.c[k];
+                const int64_t pk = SUF(pair_index)(c->type, c->ia, c->ib, nb, ns);
+                while (j < *warm->n && warm->p[j] < pk) ++j;
+                REAL sums[3] = {0, 0, 0};
+                if (j < *warm->n && warm->p[j] == pk)
+                    for (int q = 0; q < 3; ++q) sums[q] = warm->sums[3 * j + q];
+                SUF(constraint)(&w, &cl.c[k], p, sums);
+                if (k < m) {
+                    np_[k] = pk;
+                    for (int q = 0; q < 3; ++q) ns_[3 * k + q] = sums[q];
+                }
+            }
+            memcpy(warm->p, np_, sizeof(int64_t) * (size_t)m);
+            memcpy(warm->sums, ns_, sizeof(REAL) * 3 * (size_t)m);
+            *warm->n = m;
+            free(np_);
+            free(ns_);
+        }
         SUF(integrate_velocities)(&w, p);
         SUF(rebuild_models)(&w);
         st[2] += cl.n;
@@ -1225,7 +1289,7 @@ int SUF(ora_world_run)(int32_t nb, int32_t ns,
                        int64_t *stats)
 {
     return SUF(world_run)(nb, ns, pos, vel, angvel, angles, NULL, force, torque, scale, mass, moi, p, nsteps,
-                          contacts, max_contacts, ncontacts, stats);
+                          contacts, max_contacts, ncontacts, stats, NULL);
 }
 
 int SUF(ora_world_run_quat)(int32_t nb, int32_t ns,
@@ -1238,7 +1302,25 @@ int SUF(ora_world_run_quat)(int32_t nb, int32_t ns,
 {
     if (!quat) return -1;
     return SUF(world_run)(nb, ns, pos, vel, angvel, angles, quat, force, torque, scale, mass, moi, p, nsteps,
-                          contacts, max_contacts, ncontacts, stats);
+                          contacts, max_contacts, ncontacts, stats, NULL);
+}
+
+int SUF(ora_world_run_warm)(int32_t nb, int32_t ns,
+                            REAL *pos, REAL *vel, REAL *angvel, REAL *angles, REAL *quat,
+                            REAL *force, REAL *torque,
+                            const REAL *scale, const REAL *mass, const REAL *moi,
+                            const ora_step_params *p, int32_t nsteps,
+                            ORA_CONTACT_T *contacts, int32_t max_contacts, int32_t *ncontacts,
+                            int64_t *stats, int64_t *warm_pair, REAL *warm_sums, int32_t *warm_n, int32_t warm_cap)
+{
+    if (!warm_pair || !warm_sums || !warm_n || warm_cap < 0 || *warm_n < 0 || *warm_n > warm_cap) return -1;
+    Warm wm;
+    wm.p = warm_pair;
+    wm.sums = warm_sums;
+    wm.n = warm_n;
+    wm.cap = warm_cap;
+    return SUF(world_run)(nb, ns, pos, vel, angvel, angles, quat, force, torque, scale, mass, moi, p, nsteps,
+                          contacts, max_contacts, ncontacts, stats, &wm);
 }
 
 /* ---------------------------------------------------------------- frozen-manifold rollouts */
@@ -1290,7 +1372,7 @@ int SUF(ora_frozen_record)(int32_t nb, int32_t ns,
         }
         step_count[s] = cl.n;
         for (int c = 0; c < cl.n; ++c)
-            SUF(constraint)(&w, &cl.c[c], p);
+            SUF(constraint)(&w, &cl.c[c], p, NULL);
         SUF(integrate_velocities)(&w, p);
         SUF(rebuild_models)(&w);
     }
@@ -1352,7 +1434,7 @@ int SUF(ora_frozen_run)(int32_t nb, int32_t ns,
             REAL dl[3];
             SUF(constraint_solve)(SUF(slot_body)(&w, f->slot_a), SUF(slot_body)(&w, f->slot_b), f->slot_b >= 0,
                                   mk3(f->n[0], f->n[1], f->n[2]), mk3(f->r1[0], f->r1[1], f->r1[2]),
-                                  mk3(f->r2[0], f->r2[1], f->r2[2]), p, (REAL)friction, (REAL)restitution, dl);
+                                  mk3(f->r2[0], f->r2[1], f->r2[2]), p, (REAL)friction, (REAL)restitution, dl, NULL);
         }
         for (int i = 0; i < nb + ns; ++i) {  /* IntegrateVelocities */
             Body *b = SUF(slot_body)(&w, i);

@@ -139,9 +139,16 @@ class WorldState:
         self.scale, self.mass, self.moi = cp(scene.scale), cp(scene.mass), cp(scene.moi)
         self.params = scene.params
         self.quat = None  # [n][4] (w, x, y, z) in the opt-in quaternion mode
+        self.warm = None  # the opt-in warm start's contact list (enable_warm)
         self.friction = None if scene.world_friction is None else float(scene.world_friction[w])
         self.restitution = None if scene.world_restitution is None else float(scene.world_restitution[w])
         self.stats = np.zeros(8, np.int64)
+
+    def enable_warm(self, cap: int):
+        """Opt-in warm start: carry each contact's accumulators to the next step
+        (at most cap contacts persist); the list starts empty."""
+        ft = np.float32 if self.precision == "f32" else np.float64
+        self.warm = (np.zeros(cap, np.int64), np.zeros((cap, 3), ft), np.zeros(1, np.int32), int(cap))
 
     def run(self, nsteps: int, contacts: bool = False, max_contacts: int = 1 << 16, opt: str = "O2"):
         """Advance nsteps; returns the contact list of the last step if asked."""
@@ -149,7 +156,13 @@ class WorldState:
         cbuf = np.zeros(max_contacts, CONTACT_DTYPE[self.precision]) if contacts else None
         ncon = ctypes.c_int32(0)
         state = [_ptr(self.pos), _ptr(self.vel), _ptr(self.angvel), _ptr(self.angles)]
-        if self.quat is None:
+        extra = []
+        if self.warm is not None:
+            fn = getattr(lib(opt), "ora_world_run_warm_" + self.precision)
+            state.append(_ptr(self.quat))
+            wp, wsum, wn, cap = self.warm
+            extra = [_ptr(wp), _ptr(wsum), _ptr(wn), cap]
+        elif self.quat is None:
             fn = getattr(lib(opt), "ora_world_run_" + self.precision)
         else:
             fn = getattr(lib(opt), "ora_world_run_quat_" + self.precision)
@@ -157,7 +170,7 @@ class WorldState:
         rc = fn(self.nb, self.ns, *state,
                 _ptr(self.force), _ptr(self.torque), _ptr(self.scale), _ptr(self.mass), _ptr(self.moi),
                 ctypes.byref(prm), int(nsteps), _ptr(cbuf), int(max_contacts), ctypes.byref(ncon),
-                _ptr(self.stats))
+                _ptr(self.stats), *extra)
         if rc != 0:
             raise ValueError("oracle world_run rejected its arguments")
         if contacts:

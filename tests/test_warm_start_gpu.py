@@ -1,0 +1,92 @@
+"""The opt-in warm start of the plain step on the GPU against the oracle's warm
+start, bit for bit: config 1 step by step, config-4 worlds with their own
+coefficients, the reference demo scene; the lists emptied by set_state; the
+batched step and the differentiable rollout refusing the mode."""
+import numpy as np
+import pytest
+
+from nans_projekat_amd import runtime
+from nans_projekat_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("pos", "vel", "angvel", "angles")
+
+
+def _warm_ctx(sc):
+    ctx = runtime.Context(sc.params)
+    ctx.upload(sc)
+    ctx.record_contacts(True)
+    ctx.set_warm_start(True)
+    return ctx
+
+
+def _oracle(oracle_mod, sc, w):
+    ws = oracle_mod.WorldState(sc, w, "f32")
+    ws.enable_warm(16 * (int(sc.offsets[w + 1]) - int(sc.offsets[w])) + 64)  # the library's capacity
+    return ws
+
+
+def _equal(got, ws, a, b, where):
+    for f in STATE:
+        np.testing.assert_array_equal(got[f][a:b], getattr(ws, f), err_msg="%s %s" % (where, f))
+
+
+def test_config1_warm_stepwise(oracle_mod):
+    sc = S.config1()
+    ctx = _warm_ctx(sc)
+    ws = _oracle(oracle_mod, sc, 0)
+    carried = 0
+    for step in range(240):
+        ctx.step_plain(1)
+        ref = ws.run(1, contacts=True)
+        con = ctx.contacts(0)
+        assert len(con) == len(ref), step
+        for f in ("index_a", "index_b", "dl_normal", "dl_tangent1", "dl_tangent2"):
+            np.testing.assert_array_equal(con[f], ref[f], err_msg="step %d %s" % (step, f))
+        if step % 20 == 19:
+            _equal(ctx.download(), ws, 0, sc.nbodies, "step %d" % step)
+        carried = max(carried, int(ws.warm[2][0]))
+    assert carried > 0
+    # the mode changes results: the cold oracle differs
+    cold = oracle_mod.WorldState(sc, 0, "f32")
+    cold.run(240)
+    assert not np.array_equal(cold.pos, ws.pos)
+    ctx.close()
+
+
+@pytest.mark.parametrize("scene", ["config4", "demo"])
+def test_worlds_warm(oracle_mod, scene):
+    sc = S.config4(8) if scene == "config4" else S.demo_scene()
+    ctx = _warm_ctx(sc)
+    ctx.step_plain(90)
+    got = ctx.download()
+    for w in range(sc.nworlds):
+        ws = _oracle(oracle_mod, sc, w)
+        ws.run(90)
+        _equal(got, ws, int(sc.offsets[w]), int(sc.offsets[w + 1]), "world %d" % w)
+    ctx.close()
+
+
+def test_set_state_empties_the_lists(oracle_mod):
+    sc = S.config1()
+    ctx = _warm_ctx(sc)
+    ctx.step_plain(50)
+    ctx.set_state(sc)          # back to the start: every contact starts from zero again
+    ctx.step_plain(30)
+    ws = _oracle(oracle_mod, sc, 0)
+    ws.run(30)
+    _equal(ctx.download(), ws, 0, sc.nbodies, "after set_state")
+    ctx.close()
+
+
+def test_batched_and_rollout_refuse_the_mode():
+    sc = S.config4(4)
+    ctx = _warm_ctx(sc)
+    with pytest.raises(runtime.NansError, match="plain step"):
+        ctx.step_batched(1)
+    with pytest.raises(runtime.NansError, match="warm"):
+        ctx.rollout(8, 4)
+    ctx.set_warm_start(False)
+    ctx.step_batched(1)
+    ctx.close()
