@@ -90,6 +90,7 @@ __global__ __launch_bounds__(256) void k_narrowphase(int64_t n, NarrowIn a, Narr
     const int count = (int)min((int64_t)NP_RANGE, n - base);
     PairSrc src{a, b, base, n, o, overflow};
     narrow_block<256>(src, count, p, lds, sh, qidx + base, qsim + base);
+    if (threadIdx.x == 0 && sh.stalled) atomicAdd(overflow, (unsigned long long)sh.stalled);
 }
 
 __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)

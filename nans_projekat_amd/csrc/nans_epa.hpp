@@ -977,15 +977,26 @@ __device__ inline EpaResult epa_group(EpgLds &L, const EpaShapes &sh, V3 sa0, V3
 //        ends takes the next candidate at once (work is handed out per wave by
 //        one LDS atomic), so a wave is not held by its slowest lane;
 //   EPA  the GJK hits are queued with their simplex; each wave takes queued hits
-//        one at a time (LDS atomic) and runs ResolveCollision on them with the
-//        polytope in its LDS block.
+//        (LDS atomic) and runs ResolveCollision on them with the polytope in its
+//        LDS block.  In the reference's arithmetic a wave starts on the queue as
+//        soon as its own GJK lanes are done, while other waves still run GJK: a
+//        hit's queue slot is published with a release store of its candidate
+//        index, and a wave that takes a slot not yet published waits for it, or
+//        for every wave to leave GJK (then the slot will never be filled).
 // Results go through the source object, per candidate, so their order is that
 // of the candidates whatever the order of the work.
 struct NarrowShared {
     int next;      // next candidate to hand out
     int nepa;      // GJK hits queued
     int epa_next;  // next queued hit to hand out
+    int gjk_done;  // waves that have left the GJK phase
+    int stalled;   // queue waits that hit their bound (counted as EPA overflow)
 };
+
+// Bound of a wave's wait for a queue slot (each round sleeps ~128 cycles): far
+// beyond any GJK phase, so reaching it means a fault, which is counted and the
+// slot skipped rather than waited on for ever.
+constexpr int EPA_WAIT_ROUNDS = 1 << 20;
 
 // A queued GJK hit's simplex: the supports on A and B of its four vertices; the
 // Minkowski point is recomputed as SupA - SupB, the expression that made it.
@@ -1012,7 +1023,11 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
         sh.next = 0;
         sh.nepa = 0;
         sh.epa_next = 0;
+        sh.gjk_done = 0;
+        sh.stalled = 0;
     }
+    if (!np.fp64_islands)
+        for (int k = threadIdx.x; k < ncand; k += BLOCK) qidx[k] = -1;  // queue slots: unpublished
     __syncthreads();
     {
         // GJK cycles: EvolveSimplex's next call depends only on the simplex's
@@ -1020,8 +1035,8 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
         // bit the calls repeat for ever and the loop can only end at the cap.  Each
         // lane keeps one earlier state (Brent's scheme: saved after calls 1, 2, 4,
         // 8, ...) in the EPA LDS, which is idle until the GJK phase ends.
-        float *cyc = reinterpret_cast<float *>(epa);  // [13][BLOCK]: P of s0..s3, n
-        const int tid = threadIdx.x;
+        float *cyc = reinterpret_cast<float *>(&epa[threadIdx.x >> 6]);  // the wave's own: [13][64], P of s0..s3, n
+        const int tid = lane;
         Shape A, B;
         Simplex S;
         S.n = 0;
@@ -1058,16 +1073,16 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
                                           S.s2.P.x, S.s2.P.y, S.s2.P.z, S.s3.P.x, S.s3.P.y, S.s3.P.z,
                                           __int_as_float(S.n)};
                     if (saved) {
-                        bool same = __float_as_int(cyc[12 * BLOCK + tid]) == S.n;
+                        bool same = __float_as_int(cyc[12 * 64 + tid]) == S.n;
 #pragma unroll
                         for (int k = 0; k < 12; ++k)
-                            same = same && (k >= 3 * S.n || __float_as_int(cyc[k * BLOCK + tid]) == __float_as_int(st[k]));
+                            same = same && (k >= 3 * S.n || __float_as_int(cyc[k * 64 + tid]) == __float_as_int(st[k]));
                         if (same) res = GJK_CAP;  // a repeated state: the cap is certain
                     }
                     ++lam;
                     if (res == GJK_STILL && (!saved || lam == power)) {
 #pragma unroll
-                        for (int k = 0; k < 13; ++k) cyc[k * BLOCK + tid] = st[k];
+                        for (int k = 0; k < 13; ++k) cyc[k * 64 + tid] = st[k];
                         power = saved ? 2 * power : 1;
                         saved = true;
                         lam = 0;
@@ -1078,34 +1093,64 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
                     src.gjk_result(c, g);
                     if (g == GJK_FOUND) {
                         const int q = atomicAdd(&sh.nepa, 1);
-                        qidx[q] = c;
                         simplex_rec_put(qsim[q], S.s0, 0);
                         simplex_rec_put(qsim[q], S.s1, 1);
                         simplex_rec_put(qsim[q], S.s2, 2);
                         simplex_rec_put(qsim[q], S.s3, 3);
+                        // publish the slot after its simplex
+                        __hip_atomic_store(&qidx[q], c, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
                     c = -1;
                 }
             }
         }
+        // this wave's queue slots are published before it counts itself out
+        if (lane == 0) __hip_atomic_fetch_add(&sh.gjk_done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-    __syncthreads();
-    const int nepa = sh.nepa;
     EpaLds &L = epa[threadIdx.x >> 6];
     if (!np.fp64_islands) {
-        // four pairs per wave, one per 16-lane group; a spilled polytope is redone
-        // by the whole wave
+        // four pairs per wave, one per 16-lane group, started while other waves
+        // may still be in GJK; a spilled polytope is redone by the whole wave
+        constexpr int NW = BLOCK / 64;
         EpgLds &G = *reinterpret_cast<EpgLds *>(&L);
         const int j = lane & 15;
         for (;;) {
             int base = 0;
             if (lane == 0) base = atomicAdd(&sh.epa_next, 4);
             base = __builtin_amdgcn_readfirstlane(base);
-            if (base >= nepa) break;
+            // the candidate of this group's slot, or -1 once it is certain the slot
+            // stays empty (every wave has left GJK and it was never published)
             const int job = base + (lane >> 4);
-            const bool act = job < nepa;
+            int c = -1;
+            if (job < ncand) {
+                for (int round = 0;; ++round) {
+                    c = __hip_atomic_load(&qidx[job], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (c >= ncand) {  // not a candidate of this block: never published here
+                        if (j == 0) atomicAdd(&sh.stalled, 1);
+                        c = -1;
+                        break;
+                    }
+                    if (c >= 0) break;
+                    if (__hip_atomic_load(&sh.gjk_done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == NW) {
+                        c = __hip_atomic_load(&qidx[job], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (c >= ncand) {
+                            if (j == 0) atomicAdd(&sh.stalled, 1);
+                            c = -1;
+                        }
+                        break;
+                    }
+                    if (round >= EPA_WAIT_ROUNDS) {
+                        if (j == 0) atomicAdd(&sh.stalled, 1);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            // slots are filled in order, so an empty first slot ends the queue
+            if (__shfl(c, 0, 64) < 0) break;
+            const bool act = c >= 0;
             const int jq = act ? job : base;
-            const int c = qidx[jq];
+            if (!act) c = __shfl(c, 0, 64);
             Shape A, B;
             src.shapes(c, A, B);
             const EpaShapes es = epa_shapes_group(A, B);
@@ -1121,9 +1166,10 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
             uint64_t spill = ballot(act && j == 0 && r.status == EPA_SPILL);
             wave_sync();
             while (spill) {
-                const int sj = base + (ffs64(spill) >> 4);
+                const int sg = ffs64(spill) >> 4;  // the spilled group
+                const int sj = base + sg;
                 spill &= spill - 1;
-                const int cs = qidx[sj];
+                const int cs = __shfl(c, 16 * sg, 64);
                 Shape SA, SB;
                 src.shapes(cs, SA, SB);
                 const EpaShapes ew = epa_shapes(SA, SB);
@@ -1146,6 +1192,8 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
         __syncthreads();
         return;
     }
+    __syncthreads();
+    const int nepa = sh.nepa;
     for (;;) {
         int job = 0;
         if (lane == 0) job = atomicAdd(&sh.epa_next, 1);

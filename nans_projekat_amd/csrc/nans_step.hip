@@ -352,7 +352,10 @@ __global__ __launch_bounds__(256) void k_plain_narrow(Planes B, const float *ver
         const int count = min(range, total - base);
         WorldSrc src{B, verts, nb, ns, &fl, cand + base, rec + base, flag + base, cnt};
         narrow_block<256>(src, count, np, lds, sh, qidx + base, qsim + base);
-        if (threadIdx.x == 0) atomicAdd(&cnt[CNT_GJK_FOUND], (unsigned long long)sh.nepa);
+        if (threadIdx.x == 0) {
+            atomicAdd(&cnt[CNT_GJK_FOUND], (unsigned long long)sh.nepa);
+            if (sh.stalled) atomicAdd(&cnt[CNT_EPA_OVERFLOW], (unsigned long long)sh.stalled);
+        }
     }
 }
 
@@ -568,7 +571,10 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgs a)
             WorldSrc src{Lb, vertsL, nb, ns, &fl, cand, rec, flag, a.cnt};
             narrow_block<BLOCK>(src, ncand, a.k.np, epa, nsh, qidx, qsim);
         }
-        if (tid == 0) atomicAdd(&a.cnt[CNT_GJK_FOUND], (unsigned long long)nsh.nepa);
+        if (tid == 0) {
+            atomicAdd(&a.cnt[CNT_GJK_FOUND], (unsigned long long)nsh.nepa);
+            if (nsh.stalled) atomicAdd(&a.cnt[CNT_EPA_OVERFLOW], (unsigned long long)nsh.stalled);
+        }
         // the contacts in candidate (= canonical) order; con reuses the queue's memory
         int ncon = 0;
         for (int base = 0; base < ncand; base += BLOCK) {
