@@ -162,24 +162,36 @@ __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int
         sb[r] = pend[r] ? con[c].slot_b : -1;
     }
     for (int c = tid + SOLVE_REG * BLOCK; c < ncon; c += BLOCK) done[c] = 0;
+    // Round k's key of contact c is (k << 21) | (2^21 - 1 - c) (contacts number
+    // below 2^21 on both paths): the largest key a body receives in round k is its
+    // earliest pending contact, and keys of earlier rounds are smaller, so head[]
+    // needs clearing only when the round number wraps (every 2047 rounds).
+    unsigned *hk = reinterpret_cast<unsigned *>(head);
+    for (int i = tid; i < n; i += BLOCK) hk[i] = 0u;
     __syncthreads();
     int pending = ncon;
+    unsigned round = 0;
     while (pending > 0) {
-        for (int i = tid; i < n; i += BLOCK) head[i] = INT_MAX;
-        __syncthreads();
+        if (++round == 2048u) {
+            for (int i = tid; i < n; i += BLOCK) hk[i] = 0u;
+            __syncthreads();
+            round = 1;
+        }
+        const unsigned rk = (round << 21) | 0x1FFFFFu;
+        auto key = [rk](int c) { return rk - (unsigned)c; };
 #pragma unroll
         for (int r = 0; r < SOLVE_REG; ++r) {
             if (pend[r]) {
-                const int c = tid + r * BLOCK;
-                atomicMin(&head[sa[r]], c);
-                if (sb[r] >= 0) atomicMin(&head[sb[r]], c);
+                const unsigned kc = key(tid + r * BLOCK);
+                atomicMax(&hk[sa[r]], kc);
+                if (sb[r] >= 0) atomicMax(&hk[sb[r]], kc);
             }
         }
         for (int c = tid + SOLVE_REG * BLOCK; c < ncon; c += BLOCK) {
             if (!done[c]) {
-                atomicMin(&head[con[c].slot_a], c);
+                atomicMax(&hk[con[c].slot_a], key(c));
                 const int b = con[c].slot_b;
-                if (b >= 0) atomicMin(&head[b], c);
+                if (b >= 0) atomicMax(&hk[b], key(c));
             }
         }
         __syncthreads();
@@ -197,7 +209,8 @@ __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int
         for (int r = 0; r < SOLVE_REG; ++r) {
             if (pend[r]) {
                 const int c = tid + r * BLOCK;
-                if (head[sa[r]] == c && (sb[r] < 0 || head[sb[r]] == c)) {
+                const unsigned kc = key(c);
+                if (hk[sa[r]] == kc && (sb[r] < 0 || hk[sb[r]] == kc)) {
                     apply(c);
                     pend[r] = false;
                 } else {
@@ -208,7 +221,7 @@ __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int
         for (int c = tid + SOLVE_REG * BLOCK; c < ncon; c += BLOCK) {
             if (!done[c]) {
                 const int a = con[c].slot_a, b = con[c].slot_b;
-                if (head[a] == c && (b < 0 || head[b] == c)) {
+                if (hk[a] == key(c) && (b < 0 || hk[b] == key(c))) {
                     apply(c);
                     done[c] = 1;
                 } else {
@@ -957,6 +970,7 @@ int nans_upload_worlds(nans_ctx *ctx, int32_t nworlds, const int32_t *world_offs
         NANS_HIP(hipMemcpy(W->wrest, world_restitution, sizeof(float) * nworlds, hipMemcpyHostToDevice));
     }
     // plain scratch: candidates of the largest world
+    static_assert(64LL * PLAIN_MAX_BODIES + 4096 < (1LL << 21), "contact numbers fit the solver's round keys");
     int64_t cap = std::min<int64_t>(maxp, 64LL * nmax + 4096);
     W->cand_cap = (int)cap;
     W->max_blk = (int)((maxp + CULL_CHUNK - 1) / CULL_CHUNK) + 1;
