@@ -998,6 +998,10 @@ struct NarrowShared {
 // slot skipped rather than waited on for ever.
 constexpr int EPA_WAIT_ROUNDS = 1 << 20;
 
+// First EvolveSimplex call whose state enters the cycle check (any start finds
+// a cycle; a later one spares the loops that end early).
+constexpr int GJK_CYCLE_START = 8;
+
 // A queued GJK hit's simplex: the supports on A and B of its four vertices; the
 // Minkowski point is recomputed as SupA - SupB, the expression that made it.
 struct SimplexRec {
@@ -1033,8 +1037,10 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
         // GJK cycles: EvolveSimplex's next call depends only on the simplex's
         // Minkowski points (SupA/SupB ride along), so once the points repeat bit for
         // bit the calls repeat for ever and the loop can only end at the cap.  Each
-        // lane keeps one earlier state (Brent's scheme: saved after calls 1, 2, 4,
-        // 8, ...) in the EPA LDS, which is idle until the GJK phase ends.
+        // lane keeps one earlier state (Brent's scheme, started at call
+        // GJK_CYCLE_START: saved after that call and then 1, 2, 4, ... calls
+        // later) in its wave's EPA LDS, which is idle until the wave's GJK ends.
+        // Most loops end before the start, and pay nothing for the check.
         float *cyc = reinterpret_cast<float *>(&epa[threadIdx.x >> 6]);  // the wave's own: [13][64], P of s0..s3, n
         const int tid = lane;
         Shape A, B;
@@ -1068,7 +1074,7 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
             if (c >= 0) {
                 int res = evolve_simplex(A, B, S, np.fp64_islands);
                 ++calls;
-                if (res == GJK_STILL && calls <= np.max_gjk_iterations) {
+                if (res == GJK_STILL && calls <= np.max_gjk_iterations && calls >= GJK_CYCLE_START) {
                     const float st[13] = {S.s0.P.x, S.s0.P.y, S.s0.P.z, S.s1.P.x, S.s1.P.y, S.s1.P.z,
                                           S.s2.P.x, S.s2.P.y, S.s2.P.z, S.s3.P.x, S.s3.P.y, S.s3.P.z,
                                           __int_as_float(S.n)};
