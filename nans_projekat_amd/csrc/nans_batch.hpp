@@ -130,6 +130,8 @@ struct nans_worlds {
     int *bp_counts = nullptr;    // [3 nmax] row counts
     int *bp_offsets = nullptr;   // [3 nmax]
     int *bp_tmp = nullptr;       // [cand_cap] unsorted rows
+    int *bp_stash = nullptr;     // [2 nmax][32] first pairs of each row
+    int *bp_ovf = nullptr;       // a row overflowed its stash
     // opt-in warm start of the plain step: two contact lists per world
     int warm_ready = 0, warm_flip = 0;
     std::vector<int64_t> h_warm_off;

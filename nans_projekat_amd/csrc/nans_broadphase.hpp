@@ -40,8 +40,10 @@
 // Output.  A found pair is credited to the canonical row of its owner (CC: the
 // smaller box, CS: the box, SS: the smaller sphere).  Rows are laid out as
 // [CC rows | CF flags | SF flags | CS rows | SS rows]; one exclusive scan of the
-// per-row counts gives every row's place in canonical order, the second pass
-// drops each pair into its row, and each row is finally sorted.
+// per-row counts gives every row's place in canonical order, and each row is
+// finally sorted into place.  The counting pass also keeps the first BP_STASH
+// pairs of every row; only when a row holds more does a second pass drop every
+// pair into its row's place.
 #pragma once
 
 namespace nans {
@@ -50,6 +52,7 @@ constexpr int BP_THREADS = 1024;
 constexpr double BP_CLAMP = 1099511627776.0;  // 2^40 cells per axis
 constexpr float BP_TAME = 1e15f;
 constexpr int BP_LEVELS = 64;
+constexpr int BP_STASH = 32;  // pairs per row kept by the counting pass
 
 struct BpGrid {
     double rho0;                // reach covered by level 0
@@ -105,8 +108,9 @@ __device__ inline float bp_wave_max(float v)
 // from the tame bodies' largest radius.
 __global__ __launch_bounds__(BP_THREADS) void k_bp_bounds(Planes B, const float *brad, int n, int *bucket_cnt,
                                                           int nbuckets, int *counts, int ncounts, BpGrid *g,
-                                                          unsigned long long *cnt, long long npairs)
+                                                          int *ovf, unsigned long long *cnt, long long npairs)
 {
+    if (threadIdx.x == 0) *ovf = 0;
     __shared__ float s_r[BP_THREADS / 64];
     for (int k = threadIdx.x; k < nbuckets; k += BP_THREADS) bucket_cnt[k] = 0;
     for (int k = threadIdx.x; k < ncounts; k += BP_THREADS) counts[k] = 0;
@@ -186,7 +190,15 @@ struct BpFind {
     int *tmp;              // rows before sorting (same offsets as cand)
     int *cand;
     int cap;
+    int *stash;            // [2 nb + ns][BP_STASH] the first pairs of every row
+    int *ovf;              // a row holds more than BP_STASH pairs
 };
+
+// Stash row of a counts index (CC rows, CS rows, SS rows; flags have none).
+__device__ inline int bp_stash_row(int idx, int nb, int ns)
+{
+    return idx < nb ? idx : (idx < 3 * nb + ns ? nb + (idx - 2 * nb - ns) : 2 * nb + (idx - 3 * nb - ns));
+}
 
 // Canonical row (index into the counts layout) and index of the pair {i, j}, and
 // its owner a and partner b in cull_keep's argument order.
@@ -225,6 +237,7 @@ __device__ inline BpPair bp_pair(int i, int j, int nb, int ns)
 template <bool WRITE>
 __global__ __launch_bounds__(256) void k_bp_find(BpFind a)
 {
+    if (WRITE && *a.ovf == 0) return;  // every row fitted the stash
     __shared__ FloorDev fl;
     if (threadIdx.x == 0) fl = *a.fl;
     __syncthreads();
@@ -238,7 +251,9 @@ __global__ __launch_bounds__(256) void k_bp_find(BpFind a)
         const BpPair q = bp_pair(i, j, nb, ns);
         if (!cull_keep(a.B.get3(F_PX, q.a), a.brad[q.a], a.B.get3(F_PX, q.b), a.brad[q.b])) return;
         if (!WRITE) {
-            atomicAdd(&a.counts[q.row], 1);
+            const int k = atomicAdd(&a.counts[q.row], 1);
+            if (k < BP_STASH) a.stash[bp_stash_row(q.row, nb, ns) * BP_STASH + k] = (int)q.p;
+            else *a.ovf = 1;
         } else {
             const int dst = a.offsets[q.row] + atomicSub(&a.counts[q.row], 1) - 1;
             if (dst < a.cap) a.tmp[dst] = (int)q.p;
@@ -293,97 +308,121 @@ __global__ __launch_bounds__(256) void k_bp_find(BpFind a)
             }
         }
     }
-    if (lane != 0) return;
+    if (WRITE || lane != 0) return;
     const bool cube = i < nb;
-    const int idx_f = cube ? nb + i : 2 * nb + (i - nb);  // CF / SF flag
-    const bool fkeep = cull_keep(a.B.get3(F_PX, i), a.brad[i], mk3(fl.c[0], fl.c[1], fl.c[2]), fl.brad);
-    if (!WRITE) {
-        a.counts[idx_f] = fkeep ? 1 : 0;
-    } else if (fkeep) {
-        const int dst = a.offsets[idx_f];
-        if (dst < a.cap) a.cand[dst] = (int)(tri(nb) + (cube ? i : nb + (i - nb)));
-    }
+    const int idx_f = cube ? nb + i : 2 * nb + (i - nb);  // CF / SF flag (written by k_bp_sort)
+    a.counts[idx_f] = cull_keep(a.B.get3(F_PX, i), a.brad[i], mk3(fl.c[0], fl.c[1], fl.c[2]), fl.brad) ? 1 : 0;
 }
 
-// Sorts every CC / CS / SS row from tmp into cand, one wave per row (rank sort:
-// the entries of a row are distinct).
+// Sorts every CC / CS / SS row into cand, from the stash or, after an overflow,
+// from tmp, one wave per row (rank sort: the entries of a row are distinct); the
+// waves after the rows write the CF / SF entries, one flag per lane.
 __global__ __launch_bounds__(256) void k_bp_sort(int nb, int ns, const int *offsets, const int *ncand, const int *tmp,
-                                                 int *cand, int cap)
+                                                 const int *stash, const int *ovf, int *cand, int cap)
 {
     const int lane = threadIdx.x & 63;
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= 2 * nb + ns) return;
+    const int nrows = 2 * nb + ns;
+    if (r >= nrows) {
+        const int f = (r - nrows) * 64 + lane;  // flag f: counts index nb + f, canonical index tri(nb) + f
+        if (f < nb + ns) {
+            const int idx = nb + f;
+            const int o = offsets[idx];
+            const int end = idx + 1 < 3 * nb + 2 * ns ? offsets[idx + 1] : *ncand;
+            if (end > o && o < cap) cand[o] = (int)(tri(nb) + f);
+        }
+        return;
+    }
     const int idx = r < nb ? r : (r < 2 * nb ? 2 * nb + ns + (r - nb) : 3 * nb + ns + (r - 2 * nb));
     const int L = 3 * nb + 2 * ns;
     const int o = offsets[idx];
     const int end = min(idx + 1 < L ? offsets[idx + 1] : *ncand, cap);
     const int e = end - o;
+    // the stash row r holds this row's entries when no row overflowed
+    const int *src = *ovf ? tmp + o : stash + (size_t)r * BP_STASH;
     for (int c = 0; c < e; c += 64) {
         const int k = c + lane;
-        const int v = k < e ? tmp[o + k] : INT_MAX;
+        const int v = k < e ? src[k] : INT_MAX;
         int rank = 0;
         for (int d = 0; d < e; d += 64) {
-            const int w = d + lane < e ? tmp[o + d + lane] : INT_MAX;
+            const int w = d + lane < e ? src[d + lane] : INT_MAX;
             for (int q = 0; q < 64; ++q) rank += __shfl(w, q, 64) < v ? 1 : 0;
         }
         if (k < e) cand[o + rank] = v;
     }
 }
 
-// Exclusive scan of n counts by one workgroup, each thread summing a contiguous
-// run of up to PER values held in registers (one round of loads instead of one
-// per 1024 values).  With total != nullptr it records the candidate count and its
-// overflow like k_scan.
+// Exclusive scan of n <= PER * SCAN_THREADS counts by one workgroup.  Element
+// e = k * SCAN_THREADS + t is loaded by thread t (coalesced); the wave-level scans of
+// all PER columns run in registers, their (column, wave) totals are scanned by one
+// wave, and every element gets its offset after two barriers.  With total !=
+// nullptr it records the candidate count and its overflow like k_scan.
 template <int PER>
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_regs(const int *counts, int n, int *offsets, int *total,
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_cols(const int *counts, int n, int *offsets, int *total,
                                                             int cap, unsigned long long *cnt)
 {
-    __shared__ int wave_tot[SCAN_THREADS / 64];
-    const int per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
-    const int b = threadIdx.x * per;
-    int v[PER];
-    int sum = 0;
+    constexpr int NWV = SCAN_THREADS / 64;
+    __shared__ int wt[PER * NWV];
+    __shared__ int s_tot;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int v[PER], x[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        v[k] = k < per && b + k < n ? counts[b + k] : 0;
-        sum += v[k];
+        const int e = k * SCAN_THREADS + t;
+        v[k] = e < n ? counts[e] : 0;
     }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int x = sum;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
+    for (int k = 0; k < PER; ++k) {
+        int y = v[k];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int u = __shfl_up(y, off, 64);
+            if (lane >= off) y += u;
+        }
+        x[k] = y;
+        if (lane == 63) wt[k * NWV + w] = y;
     }
-    if (lane == 63) wave_tot[w] = x;
     __syncthreads();
-    int woff = 0, tot = 0;
-    for (int k = 0; k < SCAN_THREADS / 64; ++k) {
-        const int t = wave_tot[k];
-        woff += k < w ? t : 0;
-        tot += t;
+    if (w == 0) {
+        int run = 0;
+        for (int base = 0; base < PER * NWV; base += 64) {
+            const int i = base + lane;
+            const int val = i < PER * NWV ? wt[i] : 0;
+            int y = val;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int u = __shfl_up(y, off, 64);
+                if (lane >= off) y += u;
+            }
+            if (i < PER * NWV) wt[i] = run + y - val;
+            run += __shfl(y, 63, 64);
+        }
+        if (lane == 0) s_tot = run;
     }
-    int run = woff + x - sum;
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        if (k < per && b + k < n) offsets[b + k] = run;
-        run += v[k];
+        const int e = k * SCAN_THREADS + t;
+        if (e < n) offsets[e] = wt[k * NWV + w] + x[k] - v[k];
     }
-    if (threadIdx.x == 0 && total) {
+    if (t == 0 && total) {
+        const int tot = s_tot;
         *total = min(tot, cap);
         atomicAdd(&cnt[CNT_CANDIDATES], (unsigned long long)tot);
         if (tot > cap) atomicAdd(&cnt[CNT_CAND_OVERFLOW], 1ull);
     }
 }
 
-// k_scan_regs for n <= 48 * SCAN_THREADS, k_scan beyond.
+// k_scan_cols for n <= 48 * SCAN_THREADS, k_scan beyond.
 inline void scan_counts(const int *counts, int n, int *offsets, int *total, int cap, unsigned long long *cnt,
                         hipStream_t s)
 {
-    if (n <= 16 * SCAN_THREADS)
-        k_scan_regs<16><<<1, SCAN_THREADS, 0, s>>>(counts, n, offsets, total, cap, cnt);
+    if (n <= 8 * SCAN_THREADS)
+        k_scan_cols<8><<<1, SCAN_THREADS, 0, s>>>(counts, n, offsets, total, cap, cnt);
+    else if (n <= 16 * SCAN_THREADS)
+        k_scan_cols<16><<<1, SCAN_THREADS, 0, s>>>(counts, n, offsets, total, cap, cnt);
     else if (n <= 48 * SCAN_THREADS)
-        k_scan_regs<48><<<1, SCAN_THREADS, 0, s>>>(counts, n, offsets, total, cap, cnt);
+        k_scan_cols<48><<<1, SCAN_THREADS, 0, s>>>(counts, n, offsets, total, cap, cnt);
     else
         k_scan<<<1, SCAN_THREADS, 0, s>>>(counts, n, offsets, total, cap, cnt);
 }

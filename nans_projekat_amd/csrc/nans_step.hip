@@ -690,7 +690,7 @@ int worlds_release(nans_ctx *ctx)
                     (void *)W->blk_counts, (void *)W->blk_offsets, (void *)W->ncand, (void *)W->queue,
                     (void *)W->bp_cell, (void *)W->bp_hslot, (void *)W->bp_sorted,
                     (void *)W->bp_bcnt, (void *)W->bp_boff, (void *)W->bp_counts, (void *)W->bp_offsets,
-                    (void *)W->bp_tmp, W->bp_grid, (void *)W->warm_p[0], (void *)W->warm_p[1], (void *)W->warm_s[0],
+                    (void *)W->bp_tmp, (void *)W->bp_stash, (void *)W->bp_ovf, W->bp_grid, (void *)W->warm_p[0], (void *)W->warm_p[1], (void *)W->warm_s[0],
                     (void *)W->warm_s[1], (void *)W->warm_n[0], (void *)W->warm_n[1], (void *)W->warm_cp,
                     (void *)W->warm_sums})
         wfree(p);
@@ -846,8 +846,8 @@ static void grid_broadphase(nans_worlds *W, const Planes &B, const float *brad, 
     const int n = nb + ns, blocks = (n + 255) / 256, L = 3 * nb + 2 * ns;
     BpGrid *g = (BpGrid *)W->bp_grid;
     const unsigned mask = (unsigned)W->bp_buckets - 1;
-    k_bp_bounds<<<1, BP_THREADS, 0, s>>>(B, brad, n, W->bp_bcnt, W->bp_buckets, W->bp_counts, L, g, W->cnt,
-                                          (long long)npairs);
+    k_bp_bounds<<<1, BP_THREADS, 0, s>>>(B, brad, n, W->bp_bcnt, W->bp_buckets, W->bp_counts, L, g, W->bp_ovf,
+                                          W->cnt, (long long)npairs);
     k_bp_cells<<<blocks, 256, 0, s>>>(B, brad, n, g, mask, W->bp_cell, W->nmax, W->bp_hslot, W->bp_bcnt);
     scan_counts(W->bp_bcnt, W->bp_buckets, W->bp_boff, nullptr, 0, nullptr, s);
     k_bp_scatter<<<blocks, 256, 0, s>>>(n, W->bp_hslot, W->nmax, W->bp_boff, W->bp_sorted);
@@ -870,12 +870,15 @@ static void grid_broadphase(nans_worlds *W, const Planes &B, const float *brad, 
     f.tmp = W->bp_tmp;
     f.cand = W->cand;
     f.cap = W->cand_cap;
+    f.stash = W->bp_stash;
+    f.ovf = W->bp_ovf;
     const int wave_blocks = (n + 3) / 4;  // one wave per body
     k_bp_find<false><<<wave_blocks, 256, 0, s>>>(f);
     scan_counts(W->bp_counts, L, W->bp_offsets, W->ncand, W->cand_cap, W->cnt, s);
     k_bp_find<true><<<wave_blocks, 256, 0, s>>>(f);
-    k_bp_sort<<<(2 * nb + ns + 3) / 4, 256, 0, s>>>(nb, ns, W->bp_offsets, W->ncand, W->bp_tmp, W->cand,
-                                                    W->cand_cap);
+    const int sort_waves = 2 * nb + ns + (nb + ns + 63) / 64;  // rows, then flags 64 per wave
+    k_bp_sort<<<(sort_waves + 3) / 4, 256, 0, s>>>(nb, ns, W->bp_offsets, W->ncand, W->bp_tmp, W->bp_stash,
+                                                   W->bp_ovf, W->cand, W->cand_cap);
 }
 
 }  // namespace nans
@@ -994,6 +997,8 @@ int nans_upload_worlds(nans_ctx *ctx, int32_t nworlds, const int32_t *world_offs
     NANS_TRY(dmalloc(&W->bp_counts, (size_t)3 * nmax));
     NANS_TRY(dmalloc(&W->bp_offsets, (size_t)3 * nmax));
     NANS_TRY(dmalloc(&W->bp_tmp, (size_t)cap));
+    NANS_TRY(dmalloc(&W->bp_stash, (size_t)2 * nmax * BP_STASH));
+    NANS_TRY(dmalloc(&W->bp_ovf, 1));
     NANS_TRY(dmalloc((BpGrid **)&W->bp_grid, 1));
     for (const void *f : {(const void *)k_plain_solve<false>, (const void *)k_plain_solve<true>})
         NANS_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
