@@ -536,10 +536,35 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgs a)
         const int64_t q0 = min(npairs, (int64_t)wv * share), q1 = min(npairs, q0 + share);
         int *wbuf = reinterpret_cast<int *>(rec) + (int64_t)wv * a.cap;  // cap ints per wave
         int mine = 0;
+        // While a 64-pair chunk lies in two box rows of 64 or more pairs, its pairs
+        // follow from the chunk's start (row ci, column cj), advanced by 64 per
+        // chunk; elsewhere (short rows at the triangle's end, the other kinds) each
+        // lane decodes its own index.
+        const int ncc = nb * (nb - 1) / 2;
+        bool fast = q0 < ncc;
+        int ci = 0, cj = 0;
+        if (fast) {
+            const PairId s0 = decode_pair32((int)q0, nb, ns);
+            ci = s0.i;
+            cj = s0.j;
+        }
         for (int64_t b = q0; b < q1; b += 64) {
             const int p = (int)(b + lane);
             bool k = false;
-            if (p < q1) {
+            fast = fast && nb - ci - 2 >= 64 && b + 64 <= ncc;
+            if (fast) {
+                if (p < q1) {
+                    const int jj = cj + lane;
+                    const int sa = jj < nb ? ci : ci + 1;
+                    const int sb = jj < nb ? jj : ci + 2 + (jj - nb);
+                    k = cull_keep(Lb.get3(F_PX, sa), bradL[sa], Lb.get3(F_PX, sb), bradL[sb]);
+                }
+                cj += 64;
+                if (cj >= nb) {
+                    cj = ci + 2 + (cj - nb);
+                    ci += 1;
+                }
+            } else if (p < q1) {
                 const PairId q = decode_pair32(p, nb, ns);
                 int sa, sb;
                 pair_slots(q, nb, sa, sb);
