@@ -665,7 +665,7 @@ struct EpgLds {
     int eids[64];
 };
 static_assert(sizeof(EpgLds) <= sizeof(EpaLds), "the grouped EPA reuses the wave's EPA LDS");
-static_assert(13 * 4 * 64 <= sizeof(EpaLds), "the GJK cycle check keeps 13 words per lane in the EPA LDS");
+static_assert(16 * 4 * 64 <= sizeof(EpaLds), "GJK keeps 16 words per lane in the EPA LDS");
 
 __device__ __forceinline__ float gpull_f(float v, int src_lane)
 {
@@ -1041,7 +1041,8 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
         // GJK_CYCLE_START: saved after that call and then 1, 2, 4, ... calls
         // later) in its wave's EPA LDS, which is idle until the wave's GJK ends.
         // Most loops end before the start, and pay nothing for the check.
-        float *cyc = reinterpret_cast<float *>(&epa[threadIdx.x >> 6]);  // the wave's own: [13][64], P of s0..s3, n
+        // the wave's own: [16][64], P of s0..s3, n, then the pair's first direction
+        float *cyc = reinterpret_cast<float *>(&epa[threadIdx.x >> 6]);
         const int tid = lane;
         Shape A, B;
         Simplex S;
@@ -1062,6 +1063,10 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
                     if (my < ncand) {
                         c = my;
                         src.shapes(c, A, B);
+                        const V3 d0 = gjk_first_dir(A, B);  // kept beside the cycle state
+                        cyc[13 * 64 + tid] = d0.x;
+                        cyc[14 * 64 + tid] = d0.y;
+                        cyc[15 * 64 + tid] = d0.z;
                         S.n = 0;
                         calls = 0;
                         saved = false;
@@ -1072,7 +1077,8 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
             }
             if (!ballot(c >= 0)) break;
             if (c >= 0) {
-                int res = evolve_simplex(A, B, S, np.fp64_islands);
+                const V3 d0 = mk3(cyc[13 * 64 + tid], cyc[14 * 64 + tid], cyc[15 * 64 + tid]);
+                int res = evolve_simplex(A, B, S, np.fp64_islands, d0);
                 ++calls;
                 if (res == GJK_STILL && calls <= np.max_gjk_iterations && calls >= GJK_CYCLE_START) {
                     const float st[13] = {S.s0.P.x, S.s0.P.y, S.s0.P.z, S.s1.P.x, S.s1.P.y, S.s1.P.z,

@@ -133,10 +133,21 @@ __device__ __forceinline__ void simplex_push(Simplex &S, const Vtx &v)
     S.n += 1;
 }
 
-// EvolveSimplex (code/nans.cpp:571-769)
-__device__ __forceinline__ int evolve_simplex(const Shape &a, const Shape &b, Simplex &S, int fp64)
+// GJK's termination test length(d) <= 0.0001f (code/nans.cpp:755-767) as one
+// compare: with a correctly rounded (hence monotone) sqrt it holds exactly when
+// dot(d, d) <= the largest binary32 whose square root rounds to <= 0.0001f, which
+// is 0x322bcc77 (9.99999994e-09); NaN fails both forms.
+constexpr unsigned GJK_LEN2_MAX_BITS = 0x322bcc77u;
+
+// The first direction of a pair's GJK, normalize(PB - PA) (code/nans.cpp:575);
+// the second is its negation.
+__device__ __forceinline__ V3 gjk_first_dir(const Shape &a, const Shape &b) { return normalize(sub(b.c, a.c)); }
+
+// EvolveSimplex (code/nans.cpp:571-769); d0 = gjk_first_dir(a, b), computed once
+// per pair by the caller.
+__device__ __forceinline__ int evolve_simplex(const Shape &a, const Shape &b, Simplex &S, int fp64, V3 d0)
 {
-    V3 dir = normalize(sub(b.c, a.c));
+    V3 dir = d0;
     if (S.n == 1) {
         dir = mul(dir, -1.0f);
     } else if (S.n == 2) {
@@ -216,7 +227,7 @@ __device__ __forceinline__ int evolve_simplex(const Shape &a, const Shape &b, Si
     if (fp64) {
         double dx = (double)dir.x, dy = (double)dir.y, dz = (double)dir.z;
         if (__builtin_sqrt((dx * dx + dy * dy) + dz * dz) <= (double)0.0001f) return GJK_NO;
-    } else if (length(dir) <= 0.0001f) {
+    } else if (dot(dir, dir) <= __uint_as_float(GJK_LEN2_MAX_BITS)) {
         return GJK_NO;
     }
     // AddSupport (code/nans.cpp:521-537): pushed whatever the test decides
@@ -233,8 +244,9 @@ __device__ inline int gjk(const Shape &a, const Shape &b, const NarrowParams &p,
     int res = GJK_STILL;
     unsigned it = 0;
     int c = 0;
+    const V3 d0 = gjk_first_dir(a, b);
     while (res == GJK_STILL && it++ <= (unsigned)p.max_gjk_iterations) {
-        res = evolve_simplex(a, b, S, p.fp64_islands);
+        res = evolve_simplex(a, b, S, p.fp64_islands, d0);
         ++c;
     }
     *calls = c;
