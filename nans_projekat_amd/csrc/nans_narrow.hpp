@@ -79,12 +79,17 @@ __device__ inline V3 support(const Shape &s, V3 d)
     return s.kind == SHAPE_BOX ? box_support(s.v, d) : sphere_support(s.c, s.r, d);
 }
 
-// CalculateSupport (code/nans.cpp:463-519)
+// CalculateSupport (code/nans.cpp:463-519).  A sphere's support along -d uses
+// normalize(-d) = -normalize(d) and c + r (-n) = c - r n, both exact in IEEE
+// arithmetic, so one normalize serves both shapes.
 __device__ inline Vtx minkowski(const Shape &a, const Shape &b, V3 d)
 {
     Vtx v;
-    v.A = support(a, d);
-    v.B = support(b, mul(d, -1.0f));
+    const bool sa = a.kind != SHAPE_BOX, sb = b.kind != SHAPE_BOX;
+    V3 n = mk3(0.0f, 0.0f, 0.0f);
+    if (sa || sb) n = normalize(d);
+    v.A = sa ? add(a.c, mul(n, a.r)) : box_support(a.v, d);
+    v.B = sb ? sub(b.c, mul(n, b.r)) : box_support(b.v, mul(d, -1.0f));
     v.P = sub(v.A, v.B);
     return v;
 }
