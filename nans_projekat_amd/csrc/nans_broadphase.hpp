@@ -104,6 +104,21 @@ __device__ inline float bp_wave_max(float v)
     return v;
 }
 
+// Inclusive sum over the 64 lanes of a wave with DPP moves (VALU, no LDS
+// crossbar): shifts by 1, 2, 4, 8 within each 16-lane row, then the last lane of
+// a row broadcast into the next row (row_bcast:15) and lane 31 into the upper half
+// (row_bcast:31).  Lanes a move does not reach add 0 (bound_ctrl off, old = 0).
+__device__ inline int wave_incl_sum(int x)
+{
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return x;
+}
+
 // One workgroup: clears the bucket and row counters and sets the level-0 reach
 // from the tame bodies' largest radius.
 __global__ __launch_bounds__(BP_THREADS) void k_bp_bounds(Planes B, const float *brad, int n, int *bucket_cnt,
@@ -283,12 +298,7 @@ __global__ __launch_bounds__(256) void k_bp_find(BpFind a)
                 rs = a.bucket_off[h];
                 rc = a.bucket_cnt[h];
             }
-            int incl = rc;  // inclusive scan of the cells' entry counts over the lanes
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int t = __shfl_up(incl, off, 64);
-                if (lane >= off) incl += t;
-            }
+            const int incl = wave_incl_sum(rc);  // inclusive scan of the cells' entry counts
             const int total = __shfl(incl, 63, 64);
             for (int base = 0; base < total; base += 64) {
                 const int t = base + lane;
@@ -373,12 +383,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_cols(const int *counts, i
     }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        int y = v[k];
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int u = __shfl_up(y, off, 64);
-            if (lane >= off) y += u;
-        }
+        const int y = wave_incl_sum(v[k]);
         x[k] = y;
         if (lane == 63) wt[k * NWV + w] = y;
     }
@@ -388,12 +393,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_cols(const int *counts, i
         for (int base = 0; base < PER * NWV; base += 64) {
             const int i = base + lane;
             const int val = i < PER * NWV ? wt[i] : 0;
-            int y = val;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int u = __shfl_up(y, off, 64);
-                if (lane >= off) y += u;
-            }
+            const int y = wave_incl_sum(val);
             if (i < PER * NWV) wt[i] = run + y - val;
             run += __shfl(y, 63, 64);
         }
