@@ -18,6 +18,8 @@ Also reported on the same line:
                   grid broadphase, and with the all-pairs cull for comparison;
   differentiable  config 4 (1024 x 16 cubes, 240 steps): checkpointed rollout
                   plus the hand-written adjoint, body-steps/s;
+  opt_in_modes    the quaternion orientation (config 3, batched) and the warm start
+                  (config 5, plain) beside the reference's modes on the same steps;
   roofline        the batched step kernel: algorithmic FP32 flops per launch
                   (event counts frozen in tests/golden/work_counts.json times the
                   per-event weights below) over its average launch time, timed
@@ -140,6 +142,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--headline-only", action="store_true", help="only the batched config-3 measurement")
     ap.add_argument("--large-steps", type=int, default=480, help="config-5 steps of the large-world leg")
+    ap.add_argument("--mode-steps", type=int, default=60, help="steps of each opt-in mode measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device 3 times (scales rocprof FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -307,6 +310,37 @@ def main():
                                              "16 steps; gradients w.r.t. mass, moi, friction, restitution and the "
                                              "initial state" % dif.nsteps}
         dctx.close()
+        # opt-in modes: quaternion orientation on the batched config-3 step, warm start on
+        # the config-5 plain step (each against the reference's mode on the same steps)
+        modes = {}
+        qctx = runtime.Context(sc.params)
+        qctx.upload(sc)
+        for mode in ("euler", "quaternion"):
+            qctx.set_state(sc)
+            qctx.set_orientation_mode(mode)
+            qctx.step_batched(2)
+            qctx.set_state(sc)
+            qctx.step_batched(args.mode_steps)
+            modes[mode] = qctx.batched_timing()["elapsed_ms"] / args.mode_steps
+        qctx.close()
+        wctx = runtime.Context(big.params)
+        wtime = {}
+        for warm in (False, True):
+            wctx.upload(big)
+            wctx.set_warm_start(warm)
+            torch.cuda.synchronize()
+            w0 = time.perf_counter()
+            wctx.step_plain(args.mode_steps)
+            torch.cuda.synchronize()
+            wtime[warm] = 1e3 * (time.perf_counter() - w0) / args.mode_steps
+        wctx.close()
+        out["opt_in_modes"] = {
+            "quaternion_ms_per_step": round(modes["quaternion"], 4),
+            "euler_ms_per_step": round(modes["euler"], 4),
+            "quaternion_workload": "config3 batched step, %d steps, kernel time" % args.mode_steps,
+            "warm_start_ms_per_step": round(wtime[True], 4),
+            "cold_ms_per_step": round(wtime[False], 4),
+            "warm_start_workload": "config5 plain step (grid), first %d steps, wall time" % args.mode_steps}
     if not args.no_cpu:
         procs = args.cpu_procs or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds, procs, 240)
