@@ -648,23 +648,27 @@ __device__ inline EpaResult epa_wave(EpaLds &L, const EpaShapes &sh, V3 p0, cons
 }
 
 // ---------------------------------------------------------------------- four EPAs per wave
-// ResolveCollision for four pairs at once, one per 16-lane group (a row of the
-// wave): the same steps as epa_wave, with the polytope of each pair held by its
-// group -- face f and vertex v in lane f / v (P, SupA, SupB; P also in LDS for
-// gathers) -- group reductions by row-local DPP and group broadcasts by
-// ds_bpermute.  A polytope that outgrows a group (17 vertices, 17 faces, 17
-// horizon edges, or an edge key pushed twice) stops with EPA_SPILL and is redone
-// from its simplex by epa_wave, so the result is always epa_wave's.
+// ResolveCollision for 64/G pairs at once, one per G-lane group (G = 16: a row
+// of the wave; G = 32: half a wave): the same steps as epa_wave, with the
+// polytope of each pair held by its group -- face f and vertex v in lane f / v
+// (P, SupA, SupB; P also in LDS for gathers) -- group reductions by row-local DPP
+// (and one ds_bpermute across the two rows of a 32-lane group) and group
+// broadcasts by ds_bpermute.  A polytope that outgrows its group (G+1 vertices,
+// G+1 faces, G+1 horizon edges, or an edge key pushed twice) stops with EPA_SPILL
+// and is redone from its simplex by the next wider tier (16 -> 32 -> epa_wave),
+// so the result is always epa_wave's.
 constexpr int EPA_SPILL = 99;
-constexpr int EPG_MAX = 16;
 
+template <int G>
 struct EpgLds {
-    float vp[4][EPG_MAX][3];   // vertex P per group
-    float sf[4][EPG_MAX][9];   // compaction staging: n, a, d, id, cl
-    int ekey[64];              // horizon edges per group (16 each)
+    static_assert(G == 16 || G == 32, "groups of 16 or 32 lanes");
+    float vp[64 / G][G][3];   // vertex P per group
+    float sf[64 / G][G][9];   // compaction staging: n, a, d, id, cl
+    int ekey[64];             // horizon edges per group (G each)
     int eids[64];
 };
-static_assert(sizeof(EpgLds) <= sizeof(EpaLds), "the grouped EPA reuses the wave's EPA LDS");
+static_assert(sizeof(EpgLds<16>) <= sizeof(EpaLds), "the grouped EPA reuses the wave's EPA LDS");
+static_assert(sizeof(EpgLds<32>) <= sizeof(EpaLds), "the grouped EPA reuses the wave's EPA LDS");
 static_assert(16 * 4 * 64 <= sizeof(EpaLds), "GJK keeps 16 words per lane in the EPA LDS");
 
 __device__ __forceinline__ float gpull_f(float v, int src_lane)
@@ -676,19 +680,26 @@ __device__ __forceinline__ V3 gpull3(V3 v, int src_lane)
 {
     return mk3(gpull_f(v.x, src_lane), gpull_f(v.y, src_lane), gpull_f(v.z, src_lane));
 }
-// this lane's group bits of a wave ballot
-__device__ __forceinline__ uint32_t gbits(uint64_t m) { return (uint32_t)(m >> (lane_id() & 48)) & 0xffffu; }
-// minimum over the lane's row of 16 (no NaN inputs), in every lane of the row
-__device__ __forceinline__ float row_min(float v)
+// this lane's G-lane group bits of a wave ballot
+template <int G>
+__device__ __forceinline__ uint32_t gbits(uint64_t m)
+{
+    return (uint32_t)(m >> (lane_id() & (64 - G))) & (uint32_t)((1ull << G) - 1ull);
+}
+// minimum over the lane's group of G (no NaN inputs), in every lane of the group:
+// a row of 16 by DPP, then the two rows of a 32-lane group through ds_bpermute
+template <int G>
+__device__ __forceinline__ float group_min(float v)
 {
     v = fminf(v, dpp_f<0xB1>(v));
     v = fminf(v, dpp_f<0x4E>(v));
     v = fminf(v, dpp_f<0x141>(v));
     v = fminf(v, dpp_f<0x140>(v));
+    if (G == 32) v = fminf(v, gpull_f(v, lane_id() ^ 16));
     return v;
 }
 
-// The group's shapes: lanes 0-7 of each group hold the corners of A, 8-15 of B.
+// The group's shapes: lanes 0-7 of each row hold the corners of A, 8-15 of B.
 __device__ inline EpaShapes epa_shapes_group(const Shape &A, const Shape &B)
 {
     const int j = lane_id() & 15;
@@ -705,10 +716,12 @@ __device__ inline EpaShapes epa_shapes_group(const Shape &A, const Shape &B)
     return e;
 }
 
-// CalculateSupport for each group's pair along its d (group-uniform).
+// CalculateSupport for each group's pair along its d (group-uniform): the corner
+// argmax runs in every row; a G-lane group reads the result of its first row.
+template <int G>
 __device__ inline Vtx group_support(const EpaShapes &s, V3 d)
 {
-    const int lane = lane_id(), j = lane & 15, gb = lane & 48;
+    const int lane = lane_id(), j = lane & 15, gb = lane & (64 - G);
     const V3 nd = mul(d, -1.0f);
     const float dist = dot(s.k, j >= 8 ? nd : d);
     float key = dist > -FLT_MAX ? dist : -__builtin_inff();
@@ -741,11 +754,13 @@ __device__ __forceinline__ int edge_key(int cl, int e)
 
 // Every lane calls; lanes 0-3 of a group pass the supports of simplex vertex j in
 // sa0 / sb0.  act: the group has a pair.  Returns the group's result (EPA_NONE for
-// an idle group, EPA_SPILL to redo with epa_wave).
-__device__ inline EpaResult epa_group(EpgLds &L, const EpaShapes &sh, V3 sa0, V3 sb0, bool act,
+// an idle group, EPA_SPILL to redo with the next wider tier).
+template <int G>
+__device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0, V3 sb0, bool act,
                                       const NarrowParams &p)
 {
-    const int lane = lane_id(), j = lane & 15, gb = lane & 48, g = lane >> 4;
+    const int lane = lane_id(), j = lane & (G - 1), gb = lane & (64 - G), g = lane / G;
+    auto gbits = [](uint64_t m) { return nans::gbits<G>(m); };
     EpaResult r;
     r.status = act ? EPA_ITER : EPA_NONE;
     r.pa = r.pb = r.n = mk3(0.0f, 0.0f, 0.0f);
@@ -797,7 +812,7 @@ __device__ inline EpaResult epa_group(EpgLds &L, const EpaShapes &sh, V3 sa0, V3
         // closest face: the first face holding the minimum below face 0's distance
         const float d0 = gpull_f(gabs(dot(mk3(F.nx, F.ny, F.nz), mk3(F.ax, F.ay, F.az))), gb);
         const float k0 = (j < nf && F.d < d0) ? F.d : __builtin_inff();
-        const float m = row_min(k0);
+        const float m = group_min<G>(k0);
         const uint32_t b0 = gbits(ballot(k0 == m));
         const int best = (m != __builtin_inff() && b0) ? ffs64(b0) : 0;
         const int owner = gb + best;
@@ -806,14 +821,15 @@ __device__ inline EpaResult epa_group(EpgLds &L, const EpaShapes &sh, V3 sa0, V3
         const float cur = m == __builtin_inff() ? d0 : dsel;
         const int fid = gpull_i(F.id, owner);
         // new support along the stored normal
-        const Vtx ns = group_support(sh, N);
+        const Vtx ns = group_support<G>(sh, N);
         const bool conv = run && dot(N, ns.P) - cur < p.epa_tolerance;
         if (ballot(conv)) {
             // Barycentric (code/nans.cpp:771-785) of N * d on the chosen face
             const int a = fid & 255, b = (fid >> 8) & 255, c = (fid >> 16) & 255;
-            const V3 PA = gpull3(VP, gb + (a & 15)), PB = gpull3(VP, gb + (b & 15)), PC = gpull3(VP, gb + (c & 15));
-            const V3 AA = gpull3(VA, gb + (a & 15)), AB = gpull3(VA, gb + (b & 15)), AC = gpull3(VA, gb + (c & 15));
-            const V3 BA = gpull3(VB, gb + (a & 15)), BB = gpull3(VB, gb + (b & 15)), BC = gpull3(VB, gb + (c & 15));
+            const int am = gb + (a & (G - 1)), bm = gb + (b & (G - 1)), cm = gb + (c & (G - 1));
+            const V3 PA = gpull3(VP, am), PB = gpull3(VP, bm), PC = gpull3(VP, cm);
+            const V3 AA = gpull3(VA, am), AB = gpull3(VA, bm), AC = gpull3(VA, cm);
+            const V3 BA = gpull3(VB, am), BB = gpull3(VB, bm), BC = gpull3(VB, cm);
             if (conv) {
                 const V3 P = mul(N, cur);
                 const V3 e0 = sub(PB, PA), e1 = sub(PC, PA), e2 = sub(P, PA);
@@ -837,7 +853,7 @@ __device__ inline EpaResult epa_group(EpgLds &L, const EpaShapes &sh, V3 sa0, V3
                 run = false;
             }
         }
-        if (run && nv >= EPG_MAX) {
+        if (run && nv >= G) {
             r.status = EPA_SPILL;
             run = false;
         }
@@ -864,7 +880,8 @@ __device__ inline EpaResult epa_group(EpgLds &L, const EpaShapes &sh, V3 sa0, V3
         // of this lane's three edges: seen / twice for same-key and reverse matches.
         const int cl = F.cl;
         uint32_t s1 = 0, s2 = 0, r1 = 0, r2 = 0;
-        const int tmax = max(max(readlane_i(nvis, 0), readlane_i(nvis, 16)), max(readlane_i(nvis, 32), readlane_i(nvis, 48)));
+        const int tmax = G == 16 ? max(max(readlane_i(nvis, 0), readlane_i(nvis, 16)), max(readlane_i(nvis, 32), readlane_i(nvis, 48)))
+                                 : max(readlane_i(nvis, 0), readlane_i(nvis, 32));
         uint32_t rest = vis;
         for (int t = 0; t < tmax; ++t) {
             const int f = rest ? ffs64(rest) : 0;
@@ -904,7 +921,7 @@ __device__ inline EpaResult epa_group(EpgLds &L, const EpaShapes &sh, V3 sa0, V3
         const uint32_t lower = (1u << j) - 1u;
         const uint32_t p0 = gbits(ballot(cnt & 1)), p1 = gbits(ballot(cnt & 2));
         const int ne = __popc(p0) + 2 * __popc(p1);
-        if (run && (gbits(ballot(dup)) != 0u || ne > EPG_MAX)) {
+        if (run && (gbits(ballot(dup)) != 0u || ne > G)) {
             r.status = EPA_SPILL;
             run = false;
         }
@@ -933,7 +950,7 @@ __device__ inline EpaResult epa_group(EpgLds &L, const EpaShapes &sh, V3 sa0, V3
         const bool al = j < nf && !w0;
         const uint32_t alive = gbits(ballot(al));
         const int nsurv = __popc(alive);
-        if (run && nsurv + ne > EPG_MAX) {
+        if (run && nsurv + ne > G) {
             r.status = EPA_SPILL;
             run = false;
         }
@@ -1018,7 +1035,9 @@ __device__ __forceinline__ void simplex_rec_put(SimplexRec &q, const Vtx &v, int
 //   void shapes(int c, Shape &A, Shape &B) const       both shapes of candidate c
 //   void gjk_result(int c, int gjk) const               every candidate
 //   void epa_result(int c, const EpaResult &r) const    every GJK hit (lane 0 calls)
-template <int BLOCK, class Src>
+// WIDE: polytopes that outgrow a 16-lane group go to a 32-lane group before the
+// whole wave (worth its registers where many do, as among random deep overlaps).
+template <int BLOCK, bool WIDE, class Src>
 __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, EpaLds *epa, NarrowShared &sh,
                              int *qidx, SimplexRec *qsim)
 {
@@ -1122,9 +1141,12 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
     EpaLds &L = epa[threadIdx.x >> 6];
     if (!np.fp64_islands) {
         // four pairs per wave, one per 16-lane group, started while other waves
-        // may still be in GJK; a spilled polytope is redone by the whole wave
+        // may still be in GJK; a spilled polytope is redone by the whole wave, or
+        // with WIDE by a 32-lane group (two at a time) and, if it spills there
+        // too, by the whole wave
         constexpr int NW = BLOCK / 64;
-        EpgLds &G = *reinterpret_cast<EpgLds *>(&L);
+        EpgLds<16> &G = *reinterpret_cast<EpgLds<16> *>(&L);
+        EpgLds<32> &G2 = *reinterpret_cast<EpgLds<32> *>(&L);
         const int j = lane & 15;
         for (;;) {
             int base = 0;
@@ -1173,21 +1195,66 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
                 sb0 = mk3(q.b[3 * j], q.b[3 * j + 1], q.b[3 * j + 2]);
             }
             wave_sync();
-            const EpaResult r = epa_group(G, es, sa0, sb0, act, np);
+            const EpaResult r = epa_group<16>(G, es, sa0, sb0, act, np);
             if (act && j == 0 && r.status != EPA_SPILL) src.epa_result(c, r);
-            uint64_t spill = ballot(act && j == 0 && r.status == EPA_SPILL);
+            const uint64_t spill = ballot(act && j == 0 && r.status == EPA_SPILL);
             wave_sync();
-            while (spill) {
-                const int sg = ffs64(spill) >> 4;  // the spilled group
+            const int nsp = __popcll(spill);
+            for (int t = 0; WIDE && 2 * t < nsp; ++t) {
+                // spilled group k = 2t + h goes to 32-lane group h
+                const int h = lane >> 5, j2 = lane & 31;
+                const int k = 2 * t + h;
+                const bool act2 = k < nsp;
+                const int sg = nth_set_bit(spill, act2 ? k : 2 * t) >> 4;
+                const int cs = __shfl(c, 16 * sg, 64);
                 const int sj = base + sg;
-                spill &= spill - 1;
+                Shape SA, SB;
+                src.shapes(cs, SA, SB);
+                const EpaShapes e2 = epa_shapes_group(SA, SB);
+                V3 a2 = mk3(0.0f, 0.0f, 0.0f), b2 = a2;
+                if (j2 < 4) {
+                    const SimplexRec &q = qsim[sj];
+                    a2 = mk3(q.a[3 * j2], q.a[3 * j2 + 1], q.a[3 * j2 + 2]);
+                    b2 = mk3(q.b[3 * j2], q.b[3 * j2 + 1], q.b[3 * j2 + 2]);
+                }
+                wave_sync();
+                const EpaResult r2 = epa_group<32>(G2, e2, a2, b2, act2, np);
+                if (act2 && j2 == 0 && r2.status != EPA_SPILL) src.epa_result(cs, r2);
+                uint64_t spill2 = ballot(act2 && j2 == 0 && r2.status == EPA_SPILL);
+                wave_sync();
+                while (spill2) {
+                    const int src_lane = ffs64(spill2);  // 0 or 32
+                    spill2 &= spill2 - 1;
+                    const int cw = __shfl(cs, src_lane, 64);
+                    const int sw = __shfl(sj, src_lane, 64);
+                    Shape WA, WB;
+                    src.shapes(cw, WA, WB);
+                    const EpaShapes ew = epa_shapes(WA, WB);
+                    V3 p0 = mk3(0.0f, 0.0f, 0.0f);
+                    if (lane < 4) {
+                        const SimplexRec &q = qsim[sw];
+                        V3 a = mk3(q.a[3 * lane], q.a[3 * lane + 1], q.a[3 * lane + 2]);
+                        V3 b = mk3(q.b[3 * lane], q.b[3 * lane + 1], q.b[3 * lane + 2]);
+                        p0 = sub(a, b);
+                        st3(L.va, lane, a);
+                        st3(L.vb, lane, b);
+                        st3(L.vp, lane, p0);
+                    }
+                    wave_sync();
+                    const EpaResult rw = epa_wave(L, ew, p0, np);
+                    if (lane == 0) src.epa_result(cw, rw);
+                    wave_sync();
+                }
+            }
+            for (uint64_t sp = WIDE ? 0ull : spill; sp; sp &= sp - 1) {
+                const int sg = ffs64(sp) >> 4;  // the spilled group
                 const int cs = __shfl(c, 16 * sg, 64);
                 Shape SA, SB;
                 src.shapes(cs, SA, SB);
                 const EpaShapes ew = epa_shapes(SA, SB);
                 V3 p0 = mk3(0.0f, 0.0f, 0.0f);
                 if (lane < 4) {
-                    const SimplexRec &q = qsim[sj];
+                    const SimplexRec &q = qsim[base + sg];
                     V3 a = mk3(q.a[3 * lane], q.a[3 * lane + 1], q.a[3 * lane + 2]);
                     V3 b = mk3(q.b[3 * lane], q.b[3 * lane + 1], q.b[3 * lane + 2]);
                     p0 = sub(a, b);

@@ -364,7 +364,7 @@ __global__ __launch_bounds__(256) void k_plain_narrow(Planes B, const float *ver
     for (int base = blockIdx.x * range; base < total; base += gridDim.x * range) {
         const int count = min(range, total - base);
         WorldSrc src{B, verts, nb, ns, &fl, cand + base, rec + base, flag + base, cnt};
-        narrow_block<256>(src, count, np, lds, sh, qidx + base, qsim + base);
+        narrow_block<256, false>(src, count, np, lds, sh, qidx + base, qsim + base);
         if (threadIdx.x == 0) {
             atomicAdd(&cnt[CNT_GJK_FOUND], (unsigned long long)sh.nepa);
             if (sh.stalled) atomicAdd(&cnt[CNT_EPA_OVERFLOW], (unsigned long long)sh.stalled);
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgs a)
         // narrowphase over the candidates (GJK per lane, EPA per wave)
         {
             WorldSrc src{Lb, vertsL, nb, ns, &fl, cand, rec, flag, a.cnt};
-            narrow_block<BLOCK>(src, ncand, a.k.np, epa, nsh, qidx, qsim);
+            narrow_block<BLOCK, false>(src, ncand, a.k.np, epa, nsh, qidx, qsim);
         }
         if (tid == 0) {
             atomicAdd(&a.cnt[CNT_GJK_FOUND], (unsigned long long)nsh.nepa);
