@@ -262,7 +262,7 @@ int nans_set_broadphase(nans_ctx *ctx, int32_t mode);
  * are skipped, ms_per_step = -1), restores the state, force/torque and statistics,
  * and keeps the fastest size for later nans_step_batched / nans_rollout calls.
  * Results do not depend on the size.  nans_set_batched_block forces a size
- * (0: the default, 256). */
+ * (0: the default, 128 threads when no world holds more than 64 bodies, else 256). */
 typedef struct nans_tuning {
     int32_t threads_per_world;   /* chosen */
     int32_t steps;               /* steps timed per candidate */

@@ -1359,7 +1359,15 @@ int batched_run(nans_ctx *ctx, int nsteps, int block)
     return NANS_OK;
 }
 
-int batched_block(nans_ctx *ctx) { return ctx->worlds->tuned_block ? ctx->worlds->tuned_block : 256; }
+// Threads per world: the autotuner's or the caller's choice, else 128 for worlds
+// of at most 64 bodies (all of a batch of such worlds fit on the chip at once, and
+// a step's latency is one world's) and 256 for larger ones.
+int batched_block(nans_ctx *ctx)
+{
+    const nans_worlds *W = ctx->worlds;
+    if (W->tuned_block) return W->tuned_block;
+    return W->nmax <= 64 ? 128 : 256;
+}
 
 template <int BLOCK>
 static int batched_recorded_launch(nans_ctx *ctx, hipStream_t s, ContactRec *rec_out, const int64_t *rec_off,
