@@ -1195,9 +1195,12 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
                 sb0 = mk3(q.b[3 * j], q.b[3 * j + 1], q.b[3 * j + 2]);
             }
             wave_sync();
-            const EpaResult r = epa_group<16>(G, es, sa0, sb0, act, np);
-            if (act && j == 0 && r.status != EPA_SPILL) src.epa_result(c, r);
-            const uint64_t spill = ballot(act && j == 0 && r.status == EPA_SPILL);
+            // WIDE: a pair with a sphere goes to the 32-lane tier at once (a sphere's
+            // support gives a new vertex at every iteration, so its polytopes grow)
+            const bool sph = WIDE && (A.kind != SHAPE_BOX || B.kind != SHAPE_BOX);
+            const EpaResult r = epa_group<16>(G, es, sa0, sb0, act && !sph, np);
+            if (act && !sph && j == 0 && r.status != EPA_SPILL) src.epa_result(c, r);
+            const uint64_t spill = ballot(act && j == 0 && (sph || r.status == EPA_SPILL));
             wave_sync();
             const int nsp = __popcll(spill);
             for (int t = 0; WIDE && 2 * t < nsp; ++t) {
