@@ -28,6 +28,13 @@
  * The binary32 sin/cos of rotate are the correctly rounded values (the binary64
  * results rounded to binary32); the reference's libm sinf/cosf agree with them
  * except for about 1% of arguments, where they are one ulp off.
+ *
+ * EvolveSimplex and ResolveCollision are checked call by call through
+ * ora_trace_pair_* against properties the geometry fixes (tests/
+ * test_gjk_epa_invariants.py, with planted mistakes in tests/
+ * test_oracle_mutants.py).  Parity unpinned: exact ties between EPA face
+ * distances (resolved by the first index, the strict '<' scan) are pinned only
+ * by this restatement.
  */
 #ifndef NANS_ORACLE_H
 #define NANS_ORACLE_H
@@ -211,6 +218,45 @@ int ora_world_pairs_f32(int32_t nb, int32_t ns,
                         const float *pos, const float *angles, const float *scale,
                         const ora_step_params *p,
                         int32_t *type, int32_t *index_a, int32_t *index_b, int32_t *stats);
+
+/* Trace of one pair's CheckCollision, call by call, for the invariant tests of
+ * EvolveSimplex (code/nans.cpp:571-769) and ResolveCollision (code/nans.cpp:
+ * 787-904).  Values are widened to binary64 (exactly, from binary32). */
+#define ORA_TRACE_MAXF 128
+typedef struct ora_gjk_rec {
+    int32_t n_in, n_out, result, pushed;   /* simplex sizes; result 0 no, 1 found, 2 still */
+    double s_in[4][3];                     /* Minkowski points before the call */
+    double s_out[5][3];                    /* and after it (a pushed vertex last) */
+    double dir[3];                         /* direction of the length test and the support */
+    double sup_a[3], sup_b[3];             /* supports of the pushed vertex */
+} ora_gjk_rec;
+
+typedef struct ora_epa_rec {
+    int32_t nfaces, best, converged, nfaces_out;
+    double cur;                            /* distance of the chosen face */
+    double dir[3];                         /* its stored normal: the search direction */
+    double p_new[3], sa_new[3], sb_new[3]; /* the new support */
+    double bary[3];                        /* converged: U, V, W of the contact */
+    double face_sup[3][6];                 /* converged: SupA, SupB of the face's A, B, C */
+    double face[ORA_TRACE_MAXF][12];       /* faces before the iteration: A, B, C, stored N */
+    int32_t visible[ORA_TRACE_MAXF];       /* faces the iteration removed */
+} ora_epa_rec;
+
+/* CheckCollision of one pair (shapes as in ora_narrowphase_*) with its trace:
+ * up to gcap GJK calls into g and ecap EPA iterations into e (counts in *ng,
+ * *ne); the outcome in *gjk, *epa and the contact in normal, point_a, point_b.
+ * Returns 0, 1 if a trace did not fit (a cap or ORA_TRACE_MAXF faces), -1 on bad
+ * arguments. */
+int ora_trace_pair_f32(int32_t type_a, const float *pos_a, const float *quat_a, const float *half_a,
+                       int32_t type_b, const float *pos_b, const float *quat_b, const float *half_b,
+                       const ora_np_params *p, ora_gjk_rec *g, int32_t gcap, int32_t *ng,
+                       ora_epa_rec *e, int32_t ecap, int32_t *ne, int32_t *gjk, int32_t *epa,
+                       float *normal, float *point_a, float *point_b);
+int ora_trace_pair_f64(int32_t type_a, const double *pos_a, const double *quat_a, const double *half_a,
+                       int32_t type_b, const double *pos_b, const double *quat_b, const double *half_b,
+                       const ora_np_params *p, ora_gjk_rec *g, int32_t gcap, int32_t *ng,
+                       ora_epa_rec *e, int32_t ecap, int32_t *ne, int32_t *gjk, int32_t *epa,
+                       double *normal, double *point_a, double *point_b);
 
 /* Box corners of the Model built from a pose (translate * rotX * rotY * rotZ * scale,
  * degrees), in the order of code/nans.cpp:399-406: out [8][3]. */

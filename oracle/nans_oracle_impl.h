@@ -288,6 +288,21 @@ static Vert SUF(calc_support)(const Shape *a, const Shape *b, V3 d)
 /* ---------------------------------------------------------------- GJK */
 enum { SUF(kNo) = 0, SUF(kFound) = 1, SUF(kStill) = 2 };
 
+/* Trace of ora_trace_pair (NULL buffers: off). */
+static ora_gjk_rec *SUF(g_tg);
+static int SUF(g_tgcap), SUF(g_tgn);
+static ora_epa_rec *SUF(g_te);
+static int SUF(g_tecap), SUF(g_ten), SUF(g_tovf);
+static V3 SUF(g_tdir);
+static int SUF(g_tdir_set), SUF(g_tpushed);
+
+static void SUF(put3)(double *d, V3 v)
+{
+    d[0] = (double)v.x;
+    d[1] = (double)v.y;
+    d[2] = (double)v.z;
+}
+
 static void SUF(erase)(Vert *s, int *n, int idx)
 {
     for (int i = idx; i < *n - 1; ++i)
@@ -403,6 +418,10 @@ static int SUF(evolve_simplex)(const Shape *a, const Shape *b, Vert *S, int *n, 
     }
     /* termination test (nans.cpp:755-759); binary64 in the FP64-islands mode */
     OC(OC_EVLEN, 1);
+    if (SUF(g_tg)) {
+        SUF(g_tdir) = dir;
+        SUF(g_tdir_set) = 1;
+    }
     if (fp64) {
         double dx = (double)dir.x, dy = (double)dir.y, dz = (double)dir.z;
         if (sqrt((dx * dx + dy * dy) + dz * dz) <= (double)0.0001f)
@@ -413,7 +432,40 @@ static int SUF(evolve_simplex)(const Shape *a, const Shape *b, Vert *S, int *n, 
     /* AddSupport (nans.cpp:521-537): the vertex is pushed whatever the test says */
     Vert v = SUF(calc_support)(a, b, dir);
     S[(*n)++] = v;
+    if (SUF(g_tg))
+        SUF(g_tpushed) = 1;
     return vdot(dir, v.P) >= (REAL)0 ? SUF(kStill) : SUF(kNo);
+}
+
+/* evolve_simplex, recorded when a trace is on */
+static int SUF(evolve_call)(const Shape *a, const Shape *b, Vert *S, int *n, int fp64)
+{
+    if (!SUF(g_tg))
+        return SUF(evolve_simplex)(a, b, S, n, fp64);
+    ora_gjk_rec r;
+    memset(&r, 0, sizeof(r));
+    r.n_in = *n;
+    for (int i = 0; i < *n && i < 4; ++i)
+        SUF(put3)(r.s_in[i], S[i].P);
+    SUF(g_tdir_set) = 0;
+    SUF(g_tpushed) = 0;
+    int res = SUF(evolve_simplex)(a, b, S, n, fp64);
+    r.n_out = *n;
+    for (int i = 0; i < *n && i < 5; ++i)
+        SUF(put3)(r.s_out[i], S[i].P);
+    r.result = res == SUF(kFound) ? 1 : (res == SUF(kStill) ? 2 : 0);
+    if (SUF(g_tdir_set))
+        SUF(put3)(r.dir, SUF(g_tdir));
+    r.pushed = SUF(g_tpushed);
+    if (r.pushed) {
+        SUF(put3)(r.sup_a, S[*n - 1].SupA);
+        SUF(put3)(r.sup_b, S[*n - 1].SupB);
+    }
+    if (SUF(g_tgn) < SUF(g_tgcap))
+        SUF(g_tg)[SUF(g_tgn)++] = r;
+    else
+        SUF(g_tovf) = 1;
+    return res;
 }
 
 /* ---------------------------------------------------------------- EPA */
@@ -541,6 +593,28 @@ static int SUF(epa)(const Shape *a, const Shape *b, const Vert *S, const ora_np_
         V3 dir = p.t[best].N;
         Vert ns = SUF(calc_support)(a, b, dir);
         ++nverts;
+        ora_epa_rec *tr = NULL;
+        if (SUF(g_te)) {
+            if (SUF(g_ten) < SUF(g_tecap) && p.nt <= ORA_TRACE_MAXF) {
+                tr = &SUF(g_te)[SUF(g_ten)++];
+                memset(tr, 0, sizeof(*tr));
+                tr->nfaces = p.nt;
+                for (int i = 0; i < p.nt; ++i) {
+                    SUF(put3)(&tr->face[i][0], p.t[i].A.P);
+                    SUF(put3)(&tr->face[i][3], p.t[i].B.P);
+                    SUF(put3)(&tr->face[i][6], p.t[i].C.P);
+                    SUF(put3)(&tr->face[i][9], p.t[i].N);
+                }
+                tr->best = best;
+                tr->cur = (double)cur;
+                SUF(put3)(tr->dir, dir);
+                SUF(put3)(tr->p_new, ns.P);
+                SUF(put3)(tr->sa_new, ns.SupA);
+                SUF(put3)(tr->sb_new, ns.SupB);
+            } else {
+                SUF(g_tovf) = 1;
+            }
+        }
         if (vdot(p.t[best].N, ns.P) - cur < (REAL)np->epa_tolerance) {
             double depth = (double)cur;
             if (np->fp64_islands) {
@@ -569,6 +643,21 @@ static int SUF(epa)(const Shape *a, const Shape *b, const Vert *S, const ora_np_
             Tri *t = &p.t[best];
             REAL U, V, W;
             SUF(barycentric)(vmul(t->N, cur), t->A.P, t->B.P, t->C.P, &U, &V, &W);
+            if (tr) {
+                tr->converged = 1;
+                tr->best = best;
+                tr->cur = (double)cur;
+                tr->bary[0] = (double)U;
+                tr->bary[1] = (double)V;
+                tr->bary[2] = (double)W;
+                SUF(put3)(&tr->face_sup[0][0], t->A.SupA);
+                SUF(put3)(&tr->face_sup[0][3], t->A.SupB);
+                SUF(put3)(&tr->face_sup[1][0], t->B.SupA);
+                SUF(put3)(&tr->face_sup[1][3], t->B.SupB);
+                SUF(put3)(&tr->face_sup[2][0], t->C.SupA);
+                SUF(put3)(&tr->face_sup[2][3], t->C.SupB);
+                tr->nfaces_out = p.nt;
+            }
             if (fabs(U) > 1.0 || fabs(V) > 1.0 || fabs(W) > 1.0) {
                 result = ORA_EPA_BARY;
             } else if (!SUF(is_valid)(U) || !SUF(is_valid)(V) || !SUF(is_valid)(W)) {
@@ -584,7 +673,8 @@ static int SUF(epa)(const Shape *a, const Shape *b, const Vert *S, const ora_np_
             break;
         }
         OC(OC_EPA_F, p.nt);
-        for (int i = 0; i < p.nt;) {
+        int orig = 0;
+        for (int i = 0; i < p.nt; ++orig) {
             Tri *t = &p.t[i];
             V3 temp = vsub(ns.P, t->A.P);
             V3 ab = vsub(t->B.P, t->A.P);
@@ -602,6 +692,8 @@ static int SUF(epa)(const Shape *a, const Shape *b, const Vert *S, const ora_np_
                 if (p.ne > maxe) maxe = p.ne;
                 memmove(&p.t[i], &p.t[i + 1], sizeof(Tri) * (size_t)(p.nt - i - 1));
                 --p.nt;
+                if (tr)
+                    tr->visible[orig] = 1;
                 continue;
             }
             ++i;
@@ -613,6 +705,8 @@ static int SUF(epa)(const Shape *a, const Shape *b, const Vert *S, const ora_np_
         }
         p.ne = 0;
         if (p.nt > maxf) maxf = p.nt;
+        if (tr)
+            tr->nfaces_out = p.nt;
     }
     if (st) {
         st[ORA_STAT_EPA_ITERS] = iters;
@@ -635,7 +729,7 @@ static int SUF(check_collision)(const Shape *a, const Shape *b, const ora_np_par
     int n = 0, res = SUF(kStill);
     unsigned it = 0, calls = 0;
     while (res == SUF(kStill) && it++ <= (unsigned)np->max_gjk_iterations) {
-        res = SUF(evolve_simplex)(a, b, S, &n, np->fp64_islands);
+        res = SUF(evolve_call)(a, b, S, &n, np->fp64_islands);
         ++calls;
     }
     int g = res == SUF(kFound) ? ORA_GJK_FOUND : (res == SUF(kNo) ? ORA_GJK_NO : ORA_GJK_CAP);
@@ -711,6 +805,45 @@ int SUF(ora_narrowphase)(int64_t n,
         if (point_b) { point_b[3 * i] = o.point_b.x; point_b[3 * i + 1] = o.point_b.y; point_b[3 * i + 2] = o.point_b.z; }
     }
     return 0;
+}
+
+int SUF(ora_trace_pair)(int32_t type_a, const REAL *pos_a, const REAL *quat_a, const REAL *half_a,
+                        int32_t type_b, const REAL *pos_b, const REAL *quat_b, const REAL *half_b,
+                        const ora_np_params *p, ora_gjk_rec *g, int32_t gcap, int32_t *ng,
+                        ora_epa_rec *e, int32_t ecap, int32_t *ne, int32_t *gjk, int32_t *epa,
+                        REAL *normal, REAL *point_a, REAL *point_b)
+{
+    if (!p || !pos_a || !pos_b || !quat_a || !quat_b || !half_a || !half_b || !g || !e || gcap < 0 || ecap < 0 ||
+        !ng || !ne || !gjk || !epa || !normal || !point_a || !point_b)
+        return -1;
+    if ((type_a != ORA_SHAPE_BOX && type_a != ORA_SHAPE_SPHERE) ||
+        (type_b != ORA_SHAPE_BOX && type_b != ORA_SHAPE_SPHERE))
+        return -1;
+    Shape a, b;
+    SUF(make_shape)(&a, type_a, pos_a, quat_a, half_a);
+    SUF(make_shape)(&b, type_b, pos_b, quat_b, half_b);
+    SUF(g_tg) = g;
+    SUF(g_tgcap) = gcap;
+    SUF(g_tgn) = 0;
+    SUF(g_te) = e;
+    SUF(g_tecap) = ecap;
+    SUF(g_ten) = 0;
+    SUF(g_tovf) = 0;
+    EpaOut o;
+    memset(&o, 0, sizeof(o));
+    int32_t st[ORA_NSTAT];
+    int gr = 0;
+    SUF(check_collision)(&a, &b, p, &gr, &o, st);
+    SUF(g_tg) = NULL;
+    SUF(g_te) = NULL;
+    *ng = SUF(g_tgn);
+    *ne = SUF(g_ten);
+    *gjk = gr;
+    *epa = gr == ORA_GJK_FOUND ? st[ORA_STAT_EPA_RESULT] : ORA_EPA_NONE;
+    normal[0] = o.normal.x; normal[1] = o.normal.y; normal[2] = o.normal.z;
+    point_a[0] = o.point_a.x; point_a[1] = o.point_a.y; point_a[2] = o.point_a.z;
+    point_b[0] = o.point_b.x; point_b[1] = o.point_b.y; point_b[2] = o.point_b.z;
+    return SUF(g_tovf) ? 1 : 0;
 }
 
 /* ---------------------------------------------------------------- world step */

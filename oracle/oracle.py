@@ -124,6 +124,43 @@ def narrowphase(pairs: dict, params, precision: str = "f32", stats: bool = False
     return out
 
 
+TRACE_MAXF = 128
+GJK_REC = np.dtype([("n_in", "<i4"), ("n_out", "<i4"), ("result", "<i4"), ("pushed", "<i4"),
+                    ("s_in", "<f8", (4, 3)), ("s_out", "<f8", (5, 3)), ("dir", "<f8", 3), ("sup_a", "<f8", 3),
+                    ("sup_b", "<f8", 3)])
+EPA_REC = np.dtype([("nfaces", "<i4"), ("best", "<i4"), ("converged", "<i4"), ("nfaces_out", "<i4"),
+                    ("cur", "<f8"), ("dir", "<f8", 3), ("p_new", "<f8", 3), ("sa_new", "<f8", 3),
+                    ("sb_new", "<f8", 3), ("bary", "<f8", 3), ("face_sup", "<f8", (3, 6)),
+                    ("face", "<f8", (TRACE_MAXF, 12)), ("visible", "<i4", TRACE_MAXF)])
+
+
+def trace_pair(shape_a: tuple, shape_b: tuple, params, precision: str = "f32", gcap: int = 80, ecap: int = 80):
+    """CheckCollision of one pair with its call-by-call trace (ora_trace_pair_*).
+    A shape is (type, centre[3], quaternion w,x,y,z [4], half-extents or radius [3]).
+    Returns None when the trace did not fit, else a dict with the GJK records
+    (GJK_REC), the EPA records (EPA_REC) and the outcome."""
+    ft = np.float32 if precision == "f32" else np.float64
+    arr = lambda x: np.ascontiguousarray(x, dtype=ft)
+    a = [arr(shape_a[k]) for k in (1, 2, 3)]
+    b = [arr(shape_b[k]) for k in (1, 2, 3)]
+    g = np.zeros(gcap, GJK_REC)
+    e = np.zeros(ecap, EPA_REC)
+    ng, ne, gj, ep = (ctypes.c_int32() for _ in range(4))
+    nrm, pa, pb = np.zeros(3, ft), np.zeros(3, ft), np.zeros(3, ft)
+    prm = np_params(params)
+    rc = getattr(lib(), "ora_trace_pair_" + precision)(
+        ctypes.c_int32(int(shape_a[0])), _ptr(a[0]), _ptr(a[1]), _ptr(a[2]),
+        ctypes.c_int32(int(shape_b[0])), _ptr(b[0]), _ptr(b[1]), _ptr(b[2]), ctypes.byref(prm),
+        _ptr(g), ctypes.c_int32(gcap), ctypes.byref(ng), _ptr(e), ctypes.c_int32(ecap), ctypes.byref(ne),
+        ctypes.byref(gj), ctypes.byref(ep), _ptr(nrm), _ptr(pa), _ptr(pb))
+    if rc < 0:
+        raise ValueError("oracle trace rejected its arguments")
+    if rc == 1:
+        return None
+    return dict(gjk_recs=g[:ng.value], epa_recs=e[:ne.value], gjk=gj.value, epa=ep.value, normal=nrm,
+                point_a=pa, point_b=pb)
+
+
 class WorldState:
     """One world's mutable state for the oracle (interleaved [n][3] arrays)."""
 
