@@ -6,7 +6,9 @@ the copy with gcc and runs the call-by-call checks of test_gjk_epa_invariants on
 config-2 pairs: at least one check must fail (or the polytopes must run away,
 so that the traces no longer fit).  A mistake that reverses the new faces' winding
 (PushTriangle(ns, B, A)) makes the polytope grow without bound and is left out:
-it cannot finish in a test's time."""
+it cannot finish in a test's time.  Mistakes in Constraint's quirks (Q11-Q13:
+the subtracted body-B term, W x N, the unnormalised tangent) must fail one of the
+closed-form checks of test_constraint_quirks."""
 import os
 import shutil
 import subprocess
@@ -15,6 +17,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+import test_constraint_quirks as Q
 import test_gjk_epa_invariants as T
 
 SRC = os.path.dirname(O.__file__)
@@ -47,8 +50,19 @@ MUTANTS = {
 }
 
 
+# Constraint (code/nans.cpp:1020-1329), checked by test_constraint_quirks
+CONSTRAINT_MUTANTS = {
+    "effective_mass_adds_body_b": ("JMJn += InvI1 * vdot(RN1, RN1) - InvI2 * vdot(vneg(RN2), vneg(RN2));",
+                                   "JMJn += InvI1 * vdot(RN1, RN1) + InvI2 * vdot(vneg(RN2), vneg(RN2));"),
+    "relative_velocity_w_cross_r": ("V3 dVn = vsub(vsub(vadd(V1, vcross(W1, N)), V2), vcross(W2, N));",
+                                    "V3 dVn = vsub(vsub(vadd(V1, vcross(W1, R1)), V2), vcross(W2, R2));"),
+    "tangent_normalised": ("/* glm::normalize(T1) result discarded (nans.cpp:1133) */",
+                           "T1 = vnormalize(T1);"),
+}
+
+
 def _build(tmp, name):
-    old, new = MUTANTS[name]
+    old, new = MUTANTS[name] if name in MUTANTS else CONSTRAINT_MUTANTS[name]
     d = os.path.join(str(tmp), name)
     os.makedirs(d)
     for f in ("nans_oracle.h", "nans_oracle_impl.h", "nans_oracle_f32.c", "nans_oracle_f64.c"):
@@ -94,3 +108,15 @@ def test_the_true_oracle_passes():
 def test_planted_mistake_is_caught(tmp_path, monkeypatch, name):
     monkeypatch.setattr(O, "LIB_PATH", _build(tmp_path, name))
     assert _caught(T._params()), name
+
+
+@pytest.mark.parametrize("name", sorted(CONSTRAINT_MUTANTS))
+def test_planted_constraint_mistake_is_caught(tmp_path, monkeypatch, name):
+    monkeypatch.setattr(O, "LIB_PATH", _build(tmp_path, name))
+    failed = 0
+    for check in (Q.test_impulses_do_not_depend_on_spin, Q.test_effective_mass_signs, Q.test_unnormalised_tangents):
+        try:
+            check(O)
+        except AssertionError:
+            failed += 1
+    assert failed >= 1, name
