@@ -152,14 +152,19 @@ __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int
     const int tid = threadIdx.x;
     npre = min(npre, ncon);
     for (int c = tid; c < npre; c += BLOCK) contact_terms(B, con[c], fl, k, pre[c]);
+    // a thread's first SOLVE_REG contacts live in registers: their bodies, and
+    // the impulses they apply, written back once after the rounds (no global
+    // memory access on a round's critical path when their terms are in pre[])
     int sa[SOLVE_REG], sb[SOLVE_REG];
     bool pend[SOLVE_REG];
+    float dl[SOLVE_REG][3];
 #pragma unroll
     for (int r = 0; r < SOLVE_REG; ++r) {
         const int c = tid + r * BLOCK;
         pend[r] = c < ncon;
         sa[r] = pend[r] ? con[c].slot_a : 0;
         sb[r] = pend[r] ? con[c].slot_b : -1;
+        dl[r][0] = dl[r][1] = dl[r][2] = 0.0f;
     }
     for (int c = tid + SOLVE_REG * BLOCK; c < ncon; c += BLOCK) done[c] = 0;
     // Round k's key of contact c is (k << 21) | (2^21 - 1 - c) (contacts number
@@ -211,7 +216,19 @@ __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int
                 const int c = tid + r * BLOCK;
                 const unsigned kc = key(c);
                 if (hk[sa[r]] == kc && (sb[r] < 0 || hk[sb[r]] == kc)) {
-                    apply(c);
+                    ContactRec rr;
+                    float *sums = WARM ? warm + 3 * c : nullptr;
+                    if (c < npre) {
+                        rr.slot_a = sa[r];
+                        rr.slot_b = sb[r];
+                        contact_apply(B, rr, pre[c], k, Cf, Cr, sums);
+                    } else {
+                        rr = con[c];
+                        constraint(B, rr, fl, k, Cf, Cr, sums);
+                    }
+                    dl[r][0] = rr.dln;
+                    dl[r][1] = rr.dlt1;
+                    dl[r][2] = rr.dlt2;
                     pend[r] = false;
                 } else {
                     left = 1;
@@ -230,6 +247,15 @@ __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int
             }
         }
         pending = __syncthreads_count(left);
+    }
+#pragma unroll
+    for (int r = 0; r < SOLVE_REG; ++r) {
+        const int c = tid + r * BLOCK;
+        if (c < ncon) {
+            con[c].dln = dl[r][0];
+            con[c].dlt1 = dl[r][1];
+            con[c].dlt2 = dl[r][2];
+        }
     }
 }
 
