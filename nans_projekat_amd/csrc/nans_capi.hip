@@ -30,6 +30,7 @@ struct NarrowOutDev {
 __device__ inline Shape load_shape(const NarrowIn &s, int64_t i, int64_t n)
 {
     Shape sh;
+    sh.vm = nullptr;
     sh.kind = s.type[i];
     sh.c = mk3(s.pos[i], s.pos[n + i], s.pos[2 * n + i]);
     V3 h = mk3(s.half[i], s.half[n + i], s.half[2 * n + i]);

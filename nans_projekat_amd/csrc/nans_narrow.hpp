@@ -48,6 +48,7 @@ struct Shape {
     V3 c;
     float r;
     V3 v[8];
+    const float *vm;  // the same corners in memory ([8][3]), or null
 };
 
 struct Vtx {
@@ -71,6 +72,27 @@ __device__ inline V3 box_support(const V3 *v, V3 d)
     return r;
 }
 
+// The same argmax when the corners are also in memory: the scan over the
+// registers selects an index (two selects per corner instead of four) and the
+// winner is read back from memory.
+__device__ inline V3 box_support_shape(const Shape &s, V3 d)
+{
+    if (!s.vm) return box_support(s.v, d);
+    float maxd = -FLT_MAX;
+    int idx = -1;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float dist = dot(s.v[k], d);
+        if (dist > maxd) {
+            maxd = dist;
+            idx = k;
+        }
+    }
+    if (idx < 0) return mk3(0.0f, 0.0f, 0.0f);
+    const float *w = s.vm + 3 * idx;
+    return mk3(w[0], w[1], w[2]);
+}
+
 // GetSphereSupport (code/nans.cpp:432-438)
 __device__ inline V3 sphere_support(V3 c, float r, V3 d) { return add(c, mul(normalize(d), r)); }
 
@@ -88,8 +110,8 @@ __device__ inline Vtx minkowski(const Shape &a, const Shape &b, V3 d)
     const bool sa = a.kind != SHAPE_BOX, sb = b.kind != SHAPE_BOX;
     V3 n = mk3(0.0f, 0.0f, 0.0f);
     if (sa || sb) n = normalize(d);
-    v.A = sa ? add(a.c, mul(n, a.r)) : box_support(a.v, d);
-    v.B = sb ? sub(b.c, mul(n, b.r)) : box_support(b.v, mul(d, -1.0f));
+    v.A = sa ? add(a.c, mul(n, a.r)) : box_support_shape(a, d);
+    v.B = sb ? sub(b.c, mul(n, b.r)) : box_support_shape(b, mul(d, -1.0f));
     v.P = sub(v.A, v.B);
     return v;
 }

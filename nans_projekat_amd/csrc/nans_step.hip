@@ -65,6 +65,7 @@ __device__ inline Shape body_shape(const Planes &B, const float *verts, int slot
         s.r = 0.0f;
 #pragma unroll
         for (int k = 0; k < 8; ++k) s.v[k] = mk3(fl.v[3 * k], fl.v[3 * k + 1], fl.v[3 * k + 2]);
+        s.vm = fl.v;
     } else if (slot < nb) {
         s.kind = SHAPE_BOX;
         s.c = B.get3(F_PX, slot);
@@ -72,12 +73,14 @@ __device__ inline Shape body_shape(const Planes &B, const float *verts, int slot
         const float *v = verts + 24 * slot;
 #pragma unroll
         for (int k = 0; k < 8; ++k) s.v[k] = mk3(v[3 * k], v[3 * k + 1], v[3 * k + 2]);
+        s.vm = v;
     } else {
         s.kind = SHAPE_SPHERE;
         s.c = B.get3(F_PX, slot);
         s.r = B.at(F_SX, slot);
 #pragma unroll
         for (int k = 0; k < 8; ++k) s.v[k] = mk3(0.0f, 0.0f, 0.0f);
+        s.vm = nullptr;
     }
     return s;
 }
