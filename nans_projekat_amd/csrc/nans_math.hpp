@@ -49,8 +49,14 @@ struct M4 {
     float m[16];
 };
 
-__device__ inline float dsinf(float a) { return (float)sin((double)a); }
-__device__ inline float dcosf(float a) { return (float)cos((double)a); }
+// sin and cos in binary64, rounded to binary32, with one argument reduction
+__device__ inline void dsincosf(float a, float &s, float &c)
+{
+    double sd, cd;
+    sincos((double)a, &sd, &cd);
+    s = (float)sd;
+    c = (float)cd;
+}
 
 // glm::translate(identity, v)
 __device__ inline M4 m4_translate_identity(V3 v)
@@ -69,8 +75,8 @@ __device__ inline M4 m4_translate_identity(V3 v)
 // glm::rotate(m, a, axis) (Rodrigues form of matrix_transform.inl)
 __device__ inline M4 m4_rotate(const M4 &m, float a, V3 v)
 {
-    float c = dcosf(a);
-    float s = dsinf(a);
+    float s, c;
+    dsincosf(a, s, c);
     V3 axis = normalize(v);
     V3 temp = mul(axis, 1.0f - c);
     float R[3][3];
