@@ -97,7 +97,12 @@ __device__ inline void slot_sphere(const Planes &B, const float *brad, int slot,
 }
 
 // Candidates of a world for the narrowphase driver: canonical pair indices
-// cand[0..ncand); results as a contact flag and record per candidate.
+// cand[0..ncand); results as a contact flag and record per candidate.  SMALL:
+// the world's pair count fits 24 bits (the batched path), so pairs decode in
+// 32-bit arithmetic.
+static_assert((long long)BATCH_MAX_BODIES * BATCH_MAX_BODIES < (1LL << 24),
+              "every pair index of a batched world fits the 32-bit decode");
+template <bool SMALL>
 struct WorldSrc {
     Planes B;
     const float *verts;
@@ -109,7 +114,7 @@ struct WorldSrc {
     unsigned long long *cnt;
     __device__ void shapes(int c, Shape &A, Shape &Bs) const
     {
-        PairId q = decode_pair(cand[c], nb, ns);
+        PairId q = SMALL ? decode_pair32(cand[c], nb, ns) : decode_pair(cand[c], nb, ns);
         int sa, sb;
         pair_slots(q, nb, sa, sb);
         A = body_shape(B, verts, sa, nb, *fl);
@@ -120,7 +125,7 @@ struct WorldSrc {
     {
         if (r.status == EPA_OVERFLOW) atomicAdd(&cnt[CNT_EPA_OVERFLOW], 1ull);
         if (r.status != EPA_CONTACT) return;
-        PairId q = decode_pair(cand[c], nb, ns);
+        PairId q = SMALL ? decode_pair32(cand[c], nb, ns) : decode_pair(cand[c], nb, ns);
         ContactRec &o = rec[c];
         int sa, sb;
         pair_slots(q, nb, sa, sb);
@@ -392,7 +397,7 @@ __global__ __launch_bounds__(256) void k_plain_narrow(Planes B, const float *ver
     const int range = min(NARROW_RANGE, max(64, ((total + (int)gridDim.x - 1) / (int)gridDim.x + 63) & ~63));
     for (int base = blockIdx.x * range; base < total; base += gridDim.x * range) {
         const int count = min(range, total - base);
-        WorldSrc src{B, verts, nb, ns, &fl, cand + base, rec + base, flag + base, cnt};
+        WorldSrc<false> src{B, verts, nb, ns, &fl, cand + base, rec + base, flag + base, cnt};
         narrow_block<256, false>(src, count, np, lds, sh, qidx + base, qsim + base);
         if (threadIdx.x == 0) {
             atomicAdd(&cnt[CNT_GJK_FOUND], (unsigned long long)sh.nepa);
@@ -635,7 +640,7 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgs a)
         __syncthreads();
         // narrowphase over the candidates (GJK per lane, EPA per wave)
         {
-            WorldSrc src{Lb, vertsL, nb, ns, &fl, cand, rec, flag, a.cnt};
+            WorldSrc<true> src{Lb, vertsL, nb, ns, &fl, cand, rec, flag, a.cnt};
             narrow_block<BLOCK, false>(src, ncand, a.k.np, epa, nsh, qidx, qsim);
         }
         if (tid == 0) {

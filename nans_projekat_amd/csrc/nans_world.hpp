@@ -217,12 +217,13 @@ __device__ inline PairId decode_pair(int64_t p, int nb, int ns)
 }
 
 // decode_pair for worlds whose pair count fits 24 bits (the batched path):
-// binary32 estimates of row and quotient made exact by integer fix-ups.
-__host__ __device__ inline void tri_decode32(int p, int n, int &i, int &j)
+// binary32 estimates of row and quotient (hardware sqrt and reciprocal) made
+// exact by integer fix-ups.
+__device__ inline void tri_decode32(int p, int n, int &i, int &j)
 {
     const float b = 2.0f * (float)n - 1.0f;
     const float disc = b * b - 8.0f * (float)p;
-    int r = (int)((b - sqrtf(disc > 0.0f ? disc : 0.0f)) * 0.5f);
+    int r = (int)((b - __builtin_amdgcn_sqrtf(disc > 0.0f ? disc : 0.0f)) * 0.5f);
     r = r < 0 ? 0 : (r > n - 1 ? n - 1 : r);
     auto start = [n](int row) { return row * (2 * n - row - 1) / 2; };
     while (r + 1 < n && start(r + 1) <= p) ++r;
@@ -231,7 +232,7 @@ __host__ __device__ inline void tri_decode32(int p, int n, int &i, int &j)
     j = p - start(r) + r + 1;
 }
 
-__host__ __device__ inline PairId decode_pair32(int p, int nb, int ns)
+__device__ inline PairId decode_pair32(int p, int nb, int ns)
 {
     PairId q;
     const int ncc = nb * (nb - 1) / 2;
@@ -255,7 +256,8 @@ __host__ __device__ inline PairId decode_pair32(int p, int nb, int ns)
     p -= ns;
     if (p < nb * ns) {
         q.type = NANS_CS;
-        int i = (int)((float)p * (1.0f / (float)ns));
+        int i = (int)((float)p * __builtin_amdgcn_rcpf((float)ns));
+        i = i < 0 ? 0 : i;
         while (i * ns > p) --i;
         while ((i + 1) * ns <= p) ++i;
         q.i = i;
