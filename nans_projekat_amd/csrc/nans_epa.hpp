@@ -990,9 +990,10 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
 // ---------------------------------------------------------------------- block driver
 // CheckCollision (code/nans.cpp:906-966) over the candidates [0, ncand) of one
 // workgroup, in two phases:
-//   GJK  every lane runs one candidate's EvolveSimplex loop; a lane whose loop
-//        ends takes the next candidate at once (work is handed out per wave by
-//        one LDS atomic), so a wave is not held by its slowest lane;
+//   GJK  every lane runs one candidate's EvolveSimplex loop; lanes whose loops
+//        have ended take the next candidates together once GJK_REFILL_MIN of
+//        them wait (work is handed out per wave by one LDS atomic), so a wave is
+//        not held by its slowest lane;
 //   EPA  the GJK hits are queued with their simplex; each wave takes queued hits
 //        (LDS atomic) and runs ResolveCollision on them with the polytope in its
 //        LDS block.  In the reference's arithmetic a wave starts on the queue as
