@@ -404,24 +404,29 @@ __host__ __device__ inline void contact_apply(const Planes &B, ContactRec &c, co
         sumT2 = sums[2];
     }
     const double cf_sqrt2 = 1.4142135623730951 * (double)Cf;
-    for (int it = 0; it < k.solver_iterations; ++it) {
-        float LambdaN = (-JdVn + Bb) * t.JMJn;
-        float old = sumN;
+    // The velocities are read once (code/nans.cpp:1176-1238), so every iteration
+    // adds the same lambdas; only the last iteration's deltas are applied, so the
+    // earlier ones update the sums alone.
+    const float LambdaN = (-JdVn + Bb) * t.JMJn;
+    const float LambdaT1 = (-JdVt1) * t.JMJt1;
+    const float LambdaT2 = (-JdVt2) * t.JMJt2;
+    auto iterate = [&]() {
         sumN += LambdaN;
         if (sumN < 0.0f) sumN = 0.0f;
-        dln = sumN - old;
-        float MaxLT = (float)(cf_sqrt2 * (double)sumN);
-        float LambdaT1 = (-JdVt1) * t.JMJt1;
-        float oldT1 = sumT1;
+        const float MaxLT = (float)(cf_sqrt2 * (double)sumN);
         sumT1 += LambdaT1;
         if (sumT1 < -MaxLT) sumT1 = -MaxLT;
         if (sumT1 > MaxLT) sumT1 = MaxLT;
-        dlt1 = sumT1 - oldT1;
-        float LambdaT2 = (-JdVt2) * t.JMJt2;
-        float oldT2 = sumT2;
         sumT2 += LambdaT2;
         if (sumT2 < -MaxLT) sumT2 = -MaxLT;
         if (sumT2 > MaxLT) sumT2 = MaxLT;
+    };
+    if (k.solver_iterations > 0) {
+        for (int it = 1; it < k.solver_iterations; ++it) iterate();
+        const float oldN = sumN, oldT1 = sumT1, oldT2 = sumT2;
+        iterate();
+        dln = sumN - oldN;
+        dlt1 = sumT1 - oldT1;
         dlt2 = sumT2 - oldT2;
     }
     c.dln = dln;
