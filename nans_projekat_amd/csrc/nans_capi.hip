@@ -62,6 +62,8 @@ struct PairSrc {
     int64_t base, n;
     NarrowOutDev o;
     unsigned long long *overflow;
+    static constexpr bool kMemCorners = false;
+    __device__ void shapes_ref(int c, Shape &A, Shape &B) const { shapes(c, A, B); }
     __device__ void shapes(int c, Shape &A, Shape &B) const
     {
         A = load_shape(a, base + c, n);

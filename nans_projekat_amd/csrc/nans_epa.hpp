@@ -700,6 +700,9 @@ __device__ __forceinline__ float group_min(float v)
 }
 
 // The group's shapes: lanes 0-7 of each row hold the corners of A, 8-15 of B.
+// MEM: the shapes carry only Shape::vm (corners in memory, null for a sphere),
+// and each lane reads its one corner from there.
+template <bool MEM = false>
 __device__ inline EpaShapes epa_shapes_group(const Shape &A, const Shape &B)
 {
     const int j = lane_id() & 15;
@@ -710,9 +713,14 @@ __device__ inline EpaShapes epa_shapes_group(const Shape &A, const Shape &B)
     e.cb = B.c;
     e.ra = A.r;
     e.rb = B.r;
-    const V3 ka = pick_corner(A, j & 7), kb = pick_corner(B, j & 7);
     const bool onb = j >= 8;
-    e.k = mk3(onb ? kb.x : ka.x, onb ? kb.y : ka.y, onb ? kb.z : ka.z);
+    if (MEM) {
+        const float *vm = onb ? B.vm : A.vm;
+        e.k = vm ? mk3(vm[3 * (j & 7)], vm[3 * (j & 7) + 1], vm[3 * (j & 7) + 2]) : mk3(0.0f, 0.0f, 0.0f);
+    } else {
+        const V3 ka = pick_corner(A, j & 7), kb = pick_corner(B, j & 7);
+        e.k = mk3(onb ? kb.x : ka.x, onb ? kb.y : ka.y, onb ? kb.z : ka.z);
+    }
     return e;
 }
 
@@ -1038,6 +1046,9 @@ __device__ __forceinline__ void simplex_rec_put(SimplexRec &q, const Vtx &v, int
 }
 
 // Source interface (duck-typed):
+//   static constexpr bool kMemCorners                  box corners readable in memory:
+//   void shapes_ref(int c, Shape &A, Shape &B) const   then both shapes of candidate c
+//                                                      with Shape::vm and no v[]
 //   void shapes(int c, Shape &A, Shape &B) const       both shapes of candidate c
 //   void gjk_result(int c, int gjk) const               every candidate
 //   void epa_result(int c, const EpaResult &r) const    every GJK hit (lane 0 calls)
@@ -1192,8 +1203,9 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
             const int jq = act ? job : base;
             if (!act) c = __shfl(c, 0, 64);
             Shape A, B;
-            src.shapes(c, A, B);
-            const EpaShapes es = epa_shapes_group(A, B);
+            if constexpr (Src::kMemCorners) src.shapes_ref(c, A, B);
+            else src.shapes(c, A, B);
+            const EpaShapes es = epa_shapes_group<Src::kMemCorners>(A, B);
             V3 sa0 = mk3(0.0f, 0.0f, 0.0f), sb0 = sa0;
             if (j < 4) {
                 const SimplexRec &q = qsim[jq];

@@ -56,6 +56,29 @@ __device__ inline int block_rank(bool pred, int *wave_tot, int &total)
     return off + r;
 }
 
+// A body's shape without its corner registers: kind, centre, radius and vm.
+__device__ inline Shape body_shape_ref(const Planes &B, const float *verts, int slot, int nb, const FloorDev &fl)
+{
+    Shape s;
+    if (slot < 0) {
+        s.kind = SHAPE_BOX;
+        s.c = mk3(fl.c[0], fl.c[1], fl.c[2]);
+        s.r = 0.0f;
+        s.vm = fl.v;
+    } else if (slot < nb) {
+        s.kind = SHAPE_BOX;
+        s.c = B.get3(F_PX, slot);
+        s.r = 0.0f;
+        s.vm = verts + 24 * slot;
+    } else {
+        s.kind = SHAPE_SPHERE;
+        s.c = B.get3(F_PX, slot);
+        s.r = B.at(F_SX, slot);
+        s.vm = nullptr;
+    }
+    return s;
+}
+
 __device__ inline Shape body_shape(const Planes &B, const float *verts, int slot, int nb, const FloorDev &fl)
 {
     Shape s;
@@ -119,6 +142,15 @@ struct WorldSrc {
         pair_slots(q, nb, sa, sb);
         A = body_shape(B, verts, sa, nb, *fl);
         Bs = body_shape(B, verts, sb, nb, *fl);
+    }
+    static constexpr bool kMemCorners = true;
+    __device__ void shapes_ref(int c, Shape &A, Shape &Bs) const
+    {
+        PairId q = SMALL ? decode_pair32(cand[c], nb, ns) : decode_pair(cand[c], nb, ns);
+        int sa, sb;
+        pair_slots(q, nb, sa, sb);
+        A = body_shape_ref(B, verts, sa, nb, *fl);
+        Bs = body_shape_ref(B, verts, sb, nb, *fl);
     }
     __device__ void gjk_result(int c, int g) const { flag[c] = 0; }
     __device__ void epa_result(int c, const EpaResult &r) const
