@@ -666,6 +666,10 @@ struct EpgLds {
     float sf[64 / G][G][9];   // compaction staging: n, a, d, id, cl
     int ekey[64];             // horizon edges per group (G each)
     int eids[64];
+    // horizon: bit ca * G + cb of the directed edges (ca, cb) of the visible
+    // faces, set once seen (m1) and once seen twice (m2)
+    uint32_t m1[64 / G][G * G / 32];
+    uint32_t m2[64 / G][G * G / 32];
 };
 static_assert(sizeof(EpgLds<16>) <= sizeof(EpaLds), "the grouped EPA reuses the wave's EPA LDS");
 static_assert(sizeof(EpgLds<32>) <= sizeof(EpaLds), "the grouped EPA reuses the wave's EPA LDS");
@@ -884,36 +888,43 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
         const uint32_t vis = gbits(ballot(w0));
         const int nvis = __popc(vis);
         // horizon: edges of the visible faces in face order; an edge survives iff its
-        // reverse is not among them, when no key is pushed twice (else spill).  Bits
-        // of this lane's three edges: seen / twice for same-key and reverse matches.
+        // reverse is not among them, when no key is pushed twice (else spill).  The
+        // visible faces' directed edges between non-NaN classes (< G) go into the
+        // group's edge maps; bits of this lane's three edges: same key seen twice
+        // (s2), reverse key seen (r1) and seen twice (r2).  Edges with a NaN class
+        // never match, as vertex == never holds for them.
         const int cl = F.cl;
-        uint32_t s1 = 0, s2 = 0, r1 = 0, r2 = 0;
-        const int tmax = G == 16 ? max(max(readlane_i(nvis, 0), readlane_i(nvis, 16)), max(readlane_i(nvis, 32), readlane_i(nvis, 48)))
-                                 : max(readlane_i(nvis, 0), readlane_i(nvis, 32));
-        uint32_t rest = vis;
-        for (int t = 0; t < tmax; ++t) {
-            const int f = rest ? ffs64(rest) : 0;
-            rest &= rest - 1u;
-            const int co = gpull_i(cl, gb + f);
-            if (t < nvis) {
+        constexpr int MW = G * G / 32;
+        static_assert(MW <= G, "a group's lanes clear its edge maps");
+        if (j < MW) {
+            L.m1[g][j] = 0u;
+            L.m2[g][j] = 0u;
+        }
+        wave_sync();
+        if (w0) {
 #pragma unroll
-                for (int u = 0; u < 3; ++u) {
-                    const int kt = edge_key(co, u);
+            for (int e = 0; e < 3; ++e) {
+                const int ke = edge_key(cl, e);
+                const int ca = ke & 255, cb = ke >> 8;
+                if (ca < G && cb < G) {  // classes are < G, or CLS_NAN
+                    const int key = ca * G + cb;
+                    const uint32_t bit = 1u << (key & 31);
+                    if (atomicOr(&L.m1[g][key >> 5], bit) & bit) atomicOr(&L.m2[g][key >> 5], bit);
+                }
+            }
+        }
+        wave_sync();
+        uint32_t s2 = 0, r1 = 0, r2 = 0;
+        if (w0) {
 #pragma unroll
-                    for (int e = 0; e < 3; ++e) {
-                        const int ke = edge_key(cl, e);
-                        const int ca = ke & 255, cb = ke >> 8;
-                        const bool nan = ca == CLS_NAN || cb == CLS_NAN;
-                        const uint32_t bit = 1u << e;
-                        if (kt == ke) {
-                            s2 |= s1 & bit;
-                            s1 |= bit;
-                        }
-                        if (!nan && kt == (cb | (ca << 8))) {
-                            r2 |= r1 & bit;
-                            r1 |= bit;
-                        }
-                    }
+            for (int e = 0; e < 3; ++e) {
+                const int ke = edge_key(cl, e);
+                const int ca = ke & 255, cb = ke >> 8;
+                if (ca < G && cb < G) {
+                    const int key = ca * G + cb, rkey = cb * G + ca;
+                    s2 |= ((L.m2[g][key >> 5] >> (key & 31)) & 1u) << e;
+                    r1 |= ((L.m1[g][rkey >> 5] >> (rkey & 31)) & 1u) << e;
+                    r2 |= ((L.m2[g][rkey >> 5] >> (rkey & 31)) & 1u) << e;
                 }
             }
         }
