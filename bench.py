@@ -19,7 +19,8 @@ Also reported on the same line:
   differentiable  config 4 (1024 x 16 cubes, 240 steps): checkpointed rollout
                   plus the hand-written adjoint, body-steps/s;
   opt_in_modes    the quaternion orientation (config 3, batched) and the warm start
-                  (config 5, plain) beside the reference's modes on the same steps;
+                  (config 5, plain; config 3, batched) beside the reference's modes
+                  on the same steps;
   roofline        the batched step kernel: algorithmic FP32 flops per launch
                   (event counts frozen in tests/golden/work_counts.json times the
                   per-event weights below) over its average launch time, timed
@@ -322,6 +323,12 @@ def main():
             qctx.set_state(sc)
             qctx.step_batched(args.mode_steps)
             modes[mode] = qctx.batched_timing()["elapsed_ms"] / args.mode_steps
+        qctx.set_orientation_mode("euler")
+        qctx.set_warm_start(True)
+        qctx.step_batched(2)
+        qctx.set_state(sc)  # empties the warm-start lists
+        qctx.step_batched(args.mode_steps)
+        modes["batched_warm"] = qctx.batched_timing()["elapsed_ms"] / args.mode_steps
         qctx.close()
         wctx = runtime.Context(big.params)
         wtime = {}
@@ -338,6 +345,9 @@ def main():
             "quaternion_ms_per_step": round(modes["quaternion"], 4),
             "euler_ms_per_step": round(modes["euler"], 4),
             "quaternion_workload": "config3 batched step, %d steps, kernel time" % args.mode_steps,
+            "batched_warm_start_ms_per_step": round(modes["batched_warm"], 4),
+            "batched_warm_start_workload": "config3 batched step (Euler), %d steps, kernel time against "
+                                           "euler_ms_per_step" % args.mode_steps,
             "warm_start_ms_per_step": round(wtime[True], 4),
             "cold_ms_per_step": round(wtime[False], 4),
             "warm_start_workload": "config5 plain step (grid), first %d steps, wall time" % args.mode_steps}

@@ -237,8 +237,10 @@ int nans_get_quaternions(nans_ctx *ctx, float *quat);
  * impulse being the last iteration's delta (code/nans.cpp:1176-1238), is
  * unchanged.  At most 16 n + 64 contacts of a world carry over; the others start
  * from zero.  Enabling it, nans_set_state and nans_upload_worlds empty the
- * lists.  Results differ from the reference's by design.  The batched step, its
- * autotuner and the differentiable rollout refuse the mode. */
+ * lists.  Results differ from the reference's by design.  The plain and the
+ * batched step share the lists, so the two can alternate; the batched step then
+ * launches step by step instead of replaying its graph.  The autotuner and the
+ * differentiable rollout refuse the mode. */
 int nans_set_warm_start(nans_ctx *ctx, int32_t enable);
 
 /* Broadphase of the plain step.  The reference tests every pair of a world

@@ -52,6 +52,17 @@ struct BatchArgs {
     int *rec_count;
     const int *rec_cap;
     unsigned long long *cnt;
+    // opt-in warm start (k_batched_step<BLOCK, true> only): per world w, the
+    // previous and this step's contact lists (canonical pair index, three
+    // accumulators; ascending) at warm_off[w], warm_cap[w] entries each
+    const int *warm_p_prev = nullptr;
+    const float *warm_s_prev = nullptr;
+    const int *warm_n_prev = nullptr;
+    int *warm_p_next = nullptr;
+    float *warm_s_next = nullptr;
+    int *warm_n_next = nullptr;
+    const int64_t *warm_off = nullptr;
+    const int *warm_cap = nullptr;
 };
 
 struct AdjointTape;  // nans_adjoint.hip
@@ -132,10 +143,12 @@ struct nans_worlds {
     int *bp_tmp = nullptr;       // [cand_cap] unsorted rows
     int *bp_stash = nullptr;     // [2 nmax][32] first pairs of each row
     int *bp_ovf = nullptr;       // a row overflowed its stash
-    // opt-in warm start of the plain step: two contact lists per world
+    // opt-in warm start of the plain and batched steps: two contact lists per world
     int warm_ready = 0, warm_flip = 0;
     std::vector<int64_t> h_warm_off;
     std::vector<int> h_warm_cap;
+    int64_t *warm_offd = nullptr;           // [nworlds] h_warm_off on the device
+    int *warm_capd = nullptr;               // [nworlds] h_warm_cap on the device
     int *warm_p[2] = {nullptr, nullptr};    // canonical pair indices
     float *warm_s[2] = {nullptr, nullptr};  // [3] accumulators per contact
     int *warm_n[2] = {nullptr, nullptr};    // [nworlds] list lengths

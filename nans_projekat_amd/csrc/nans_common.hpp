@@ -63,5 +63,5 @@ struct nans_ctx {
     void *np_qsim;                   // their simplices (SimplexRec)
     int broadphase;                  // NANS_BROADPHASE_* of the plain step
     int orientation;                 // NANS_ORIENT_* of both steps
-    int warm;                        // opt-in warm start of the plain step
+    int warm;                        // opt-in warm start of the plain and batched steps
 };

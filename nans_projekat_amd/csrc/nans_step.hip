@@ -537,7 +537,9 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
 
 // ------------------------------------------------------------------ batched kernel
 
-template <int BLOCK>
+// WARM: the opt-in warm start, each contact's accumulators carried over from the
+// world's previous step (a separate instantiation; the default step has none of it).
+template <int BLOCK, bool WARM = false>
 __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgs a)
 {
     extern __shared__ float smem[];
@@ -680,21 +682,58 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgs a)
             if (nsh.stalled) atomicAdd(&a.cnt[CNT_EPA_OVERFLOW], (unsigned long long)nsh.stalled);
         }
         // the contacts in candidate (= canonical) order; con reuses the queue's memory
+        // (warm start: cand[] is compacted alongside, in place, to the contacts'
+        // canonical pair indices; block_rank's barriers order the reads before the writes)
         int ncon = 0;
         for (int base = 0; base < ncand; base += BLOCK) {
             int c = base + tid;
             bool hit = c < ncand && flag[c];
+            const int key = WARM && hit ? cand[c] : 0;
             int tot;
             int rk = block_rank<BLOCK>(hit, wave_tot, tot);
-            if (hit) con[ncon + rk] = rec[c];
+            if (hit) {
+                con[ncon + rk] = rec[c];
+                if (WARM) cand[ncon + rk] = key;
+            }
             ncon += tot;
         }
         if (tid == 0) atomicAdd(&a.cnt[CNT_CONTACTS], (unsigned long long)ncon);
         __syncthreads();
+        // warm start: each contact's accumulators from the previous step's list (zero
+        // when its pair was no contact there), held in rec[]'s memory, idle from here
+        float *wsum = reinterpret_cast<float *>(rec);  // [3 ncon]
+        if (WARM) {
+            const int64_t off = a.warm_off[w];
+            const int *pp = a.warm_p_prev + off;
+            const float *sp = a.warm_s_prev + 3 * off;
+            const int np = a.warm_n_prev[w];
+            for (int c = tid; c < ncon; c += BLOCK) {
+                const int key = cand[c];
+                int lo = 0, hi = np;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (pp[mid] < key) lo = mid + 1;
+                    else hi = mid;
+                }
+                const bool hit = lo < np && pp[lo] == key;
+                for (int q = 0; q < 3; ++q) wsum[3 * c + q] = hit ? sp[3 * lo + q] : 0.0f;
+            }
+            __syncthreads();
+        }
         const float Cf = a.wfric ? a.wfric[w] : a.k.friction;
         const float Cr = a.wrest ? a.wrest[w] : a.k.restitution;
-        solve_rounds<BLOCK>(Lb, n, con, ncon, done, headL, fl, a.k, Cf, Cr, reinterpret_cast<ContactTerms *>(epa),
-                            (int)(NW * sizeof(EpaLds) / sizeof(ContactTerms)));
+        solve_rounds<BLOCK, WARM>(Lb, n, con, ncon, done, headL, fl, a.k, Cf, Cr,
+                                  reinterpret_cast<ContactTerms *>(epa),
+                                  (int)(NW * sizeof(EpaLds) / sizeof(ContactTerms)), WARM ? wsum : nullptr);
+        if (WARM) {  // this step's list: the first warm_cap[w] contacts (contact c's sums are thread c % BLOCK's)
+            const int64_t off = a.warm_off[w];
+            const int m = min(ncon, a.warm_cap[w]);
+            for (int c = tid; c < m; c += BLOCK) {
+                a.warm_p_next[off + c] = cand[c];
+                for (int q = 0; q < 3; ++q) a.warm_s_next[3 * (off + c) + q] = wsum[3 * c + q];
+            }
+            if (tid == 0) a.warm_n_next[w] = m;
+        }
         // IntegrateVelocities and write-back
         for (int i = tid; i < n; i += BLOCK) {
             if (a.k.quat) integrate_velocities_quat(Lb, Gw, i, a.k.dt);
@@ -783,7 +822,7 @@ int worlds_release(nans_ctx *ctx)
                     (void *)W->bp_bcnt, (void *)W->bp_boff, (void *)W->bp_counts, (void *)W->bp_offsets,
                     (void *)W->bp_tmp, (void *)W->bp_stash, (void *)W->bp_ovf, W->bp_grid, (void *)W->warm_p[0], (void *)W->warm_p[1], (void *)W->warm_s[0],
                     (void *)W->warm_s[1], (void *)W->warm_n[0], (void *)W->warm_n[1], (void *)W->warm_cp,
-                    (void *)W->warm_sums})
+                    (void *)W->warm_sums, (void *)W->warm_offd, (void *)W->warm_capd})
         wfree(p);
     for (hipEvent_t ev : {W->ev_in, W->ev_out, W->ev_t0, W->ev_t1})
         if (ev) hipEventDestroy(ev);
@@ -895,7 +934,9 @@ static bool use_grid(const nans_ctx *ctx, int n)
 
 // Contact lists of the opt-in warm start, allocated on its first step: per
 // world, two lists (the previous step's and this step's) of at most 16 n + 64
-// contacts, and per-contact scratch of the size of the candidate scratch.
+// contacts, and per-contact scratch of the size of the candidate scratch (plain
+// step; the batched step keeps its per-contact scratch in its workgroup's).
+// The plain and the batched step share the lists, so the two can be mixed.
 static int warm_prepare(nans_ctx *ctx)
 {
     nans_worlds *W = ctx->worlds;
@@ -914,6 +955,10 @@ static int warm_prepare(nans_ctx *ctx)
     }
     NANS_TRY(dmalloc(&W->warm_cp, (size_t)W->cand_cap));
     NANS_TRY(dmalloc(&W->warm_sums, (size_t)3 * W->cand_cap));
+    NANS_TRY(dmalloc(&W->warm_offd, (size_t)W->nworlds));
+    NANS_TRY(dmalloc(&W->warm_capd, (size_t)W->nworlds));
+    NANS_HIP(hipMemcpy(W->warm_offd, W->h_warm_off.data(), sizeof(int64_t) * W->nworlds, hipMemcpyHostToDevice));
+    NANS_HIP(hipMemcpy(W->warm_capd, W->h_warm_cap.data(), sizeof(int) * W->nworlds, hipMemcpyHostToDevice));
     W->warm_ready = 1;
     return warm_reset(ctx);
 }
@@ -1290,15 +1335,15 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
 
 namespace nans {
 
-template <int BLOCK>
+template <int BLOCK, bool WARM = false>
 static int batched_config(nans_ctx *ctx, int *grid, size_t *lds)
 {
     nans_worlds *W = ctx->worlds;
     *lds = batch_lds_bytes(W->nmax, BLOCK);
-    NANS_HIP(hipFuncSetAttribute((const void *)k_batched_step<BLOCK>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)*lds));
+    NANS_HIP(hipFuncSetAttribute((const void *)k_batched_step<BLOCK, WARM>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)*lds));
     int per_cu = 0;
-    NANS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batched_step<BLOCK>, BLOCK, *lds));
+    NANS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batched_step<BLOCK, WARM>, BLOCK, *lds));
     NANS_REQUIRE(per_cu > 0, NANS_ERR_CAPACITY, "a world of %d bodies does not fit the LDS of one workgroup",
                  W->nmax);
     int cus = 0;
@@ -1308,13 +1353,13 @@ static int batched_config(nans_ctx *ctx, int *grid, size_t *lds)
 }
 
 // Launch shape, per-workgroup scratch and kernel arguments of the batched step.
-template <int BLOCK>
+template <int BLOCK, bool WARM = false>
 static int batched_prepare(nans_ctx *ctx, BatchArgs &a, int *grid_out, size_t *lds_out)
 {
     nans_worlds *W = ctx->worlds;
     int grid = 0;
     size_t lds = 0;
-    NANS_TRY(batched_config<BLOCK>(ctx, &grid, &lds));
+    NANS_TRY((batched_config<BLOCK, WARM>(ctx, &grid, &lds)));
     int cap = (int)std::min<int64_t>(W->max_pairs, 32LL * W->nmax + 1024);
     cap = std::max(cap, 64);
     if (grid > W->b_grid || cap != W->b_cap) {
@@ -1425,6 +1470,69 @@ int batched_run(nans_ctx *ctx, int nsteps, int block)
     return NANS_OK;
 }
 
+// The warm-started batched step: launched step by step (no graph), since each
+// step swaps the previous and next contact lists.
+template <int BLOCK>
+static int batched_warm_launches(nans_ctx *ctx, int nsteps, hipStream_t cs)
+{
+    nans_worlds *W = ctx->worlds;
+    BatchArgs a;
+    int grid = 0;
+    size_t lds = 0;
+    NANS_TRY((batched_prepare<BLOCK, true>(ctx, a, &grid, &lds)));
+    a.warm_off = W->warm_offd;
+    a.warm_cap = W->warm_capd;
+    for (int s = 0; s < nsteps; ++s) {
+        const int cur = W->warm_flip, nxt = cur ^ 1;
+        a.warm_p_prev = W->warm_p[cur];
+        a.warm_s_prev = W->warm_s[cur];
+        a.warm_n_prev = W->warm_n[cur];
+        a.warm_p_next = W->warm_p[nxt];
+        a.warm_s_next = W->warm_s[nxt];
+        a.warm_n_next = W->warm_n[nxt];
+        k_zero_i32<<<1, 64, 0, cs>>>(a.queue, 1);
+        k_batched_step<BLOCK, true><<<grid, BLOCK, lds, cs>>>(a);
+        NANS_HIP(hipGetLastError());
+        W->warm_flip = nxt;
+    }
+    W->graph_block = BLOCK;
+    W->graph_grid = grid;
+    W->graph_lds = (int)lds;
+    return NANS_OK;
+}
+
+int batched_run_warm(nans_ctx *ctx, int nsteps, int block)
+{
+    nans_worlds *W = ctx->worlds;
+    NANS_REQUIRE(W->nmax <= BATCH_MAX_BODIES, NANS_ERR_CAPACITY,
+                 "the batched step keeps a world in LDS: at most %d bodies per world (largest: %d)",
+                 BATCH_MAX_BODIES, W->nmax);
+    hipStream_t cs = ctx->own_stream;
+    if (!W->ev_in) {
+        NANS_HIP(hipEventCreateWithFlags(&W->ev_in, hipEventDisableTiming));
+        NANS_HIP(hipEventCreateWithFlags(&W->ev_out, hipEventDisableTiming));
+        NANS_HIP(hipEventCreate(&W->ev_t0));
+        NANS_HIP(hipEventCreate(&W->ev_t1));
+    }
+    NANS_TRY(warm_prepare(ctx));
+    NANS_TRY(setup_floor(ctx, ctx->stream));
+    drop_graph(W);  // the graph_* launch shape below describes these launches
+    NANS_HIP(hipEventRecord(W->ev_in, ctx->stream));
+    NANS_HIP(hipStreamWaitEvent(cs, W->ev_in, 0));
+    NANS_HIP(hipEventRecord(W->ev_t0, cs));
+    switch (block) {
+    case 64: NANS_TRY(batched_warm_launches<64>(ctx, nsteps, cs)); break;
+    case 128: NANS_TRY(batched_warm_launches<128>(ctx, nsteps, cs)); break;
+    case 512: NANS_TRY(batched_warm_launches<512>(ctx, nsteps, cs)); break;
+    default: NANS_TRY(batched_warm_launches<256>(ctx, nsteps, cs)); break;
+    }
+    NANS_HIP(hipEventRecord(W->ev_t1, cs));
+    NANS_HIP(hipEventRecord(W->ev_out, cs));
+    NANS_HIP(hipStreamWaitEvent(ctx->stream, W->ev_out, 0));
+    W->timed_steps = nsteps;
+    return NANS_OK;
+}
+
 // Threads per world: the autotuner's or the caller's choice, else 128 for worlds
 // of at most 64 bodies (all of a batch of such worlds fit on the chip at once, and
 // a step's latency is one world's) and 256 for larger ones.
@@ -1475,13 +1583,13 @@ extern "C" {
 int nans_step_batched(nans_ctx *ctx, int32_t nsteps)
 {
     NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
-    NANS_REQUIRE(!ctx->warm, NANS_ERR_STATE, "warm starting is a mode of the plain step");
     NANS_REQUIRE(nsteps >= 0, NANS_ERR_INVALID, "nsteps < 0");
     if (nsteps == 0) return NANS_OK;
     NANS_HIP(hipSetDevice(ctx->p.device));
     nans_worlds *W = ctx->worlds;
     int block = batched_block(ctx);
-    NANS_TRY(batched_run(ctx, nsteps, block));
+    if (ctx->warm) NANS_TRY(batched_run_warm(ctx, nsteps, block));
+    else NANS_TRY(batched_run(ctx, nsteps, block));
     NANS_HIP(hipStreamSynchronize(ctx->stream));
     float ms = 0.0f;
     NANS_HIP(hipEventElapsedTime(&ms, W->ev_t0, W->ev_t1));
@@ -1516,7 +1624,7 @@ int nans_set_batched_block(nans_ctx *ctx, int32_t threads_per_world)
 int nans_autotune_batched(nans_ctx *ctx, int32_t nsteps, nans_tuning *out)
 {
     NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
-    NANS_REQUIRE(!ctx->warm, NANS_ERR_STATE, "warm starting is a mode of the plain step");
+    NANS_REQUIRE(!ctx->warm, NANS_ERR_STATE, "the autotuner times the cold batched step: switch warm starting off");
     NANS_REQUIRE(nsteps >= 1, NANS_ERR_INVALID, "nsteps must be >= 1");
     nans_worlds *W = ctx->worlds;
     NANS_REQUIRE(W->nmax <= BATCH_MAX_BODIES, NANS_ERR_CAPACITY,

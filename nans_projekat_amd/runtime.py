@@ -297,7 +297,7 @@ class Context:
         check(self.lib.nans_set_batched_block(self.h, int(threads_per_world)))
 
     def set_warm_start(self, enable: bool) -> None:
-        """Opt-in warm start of the plain step (contact accumulators carried over)."""
+        """Opt-in warm start of the plain and batched steps (contact accumulators carried over)."""
         check(self.lib.nans_set_warm_start(self.h, 1 if enable else 0))
 
     ORIENTATION = {"euler": 0, "quaternion": 1}

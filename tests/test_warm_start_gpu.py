@@ -1,7 +1,9 @@
-"""The opt-in warm start of the plain step on the GPU against the oracle's warm
-start, bit for bit: config 1 step by step, config-4 worlds with their own
-coefficients, the reference demo scene; the lists emptied by set_state; the
-batched step and the differentiable rollout refusing the mode."""
+"""The opt-in warm start of the plain and the batched step on the GPU against
+the oracle's warm start, bit for bit: config 1 step by step, config-4 worlds with
+their own coefficients, the reference demo scene, size-stratified config-3 worlds
+(batched, 8 to 249 bodies); the two steps alternating on one set of
+contact lists; the lists emptied by set_state; the autotuner and the
+differentiable rollout refusing the mode."""
 import numpy as np
 import pytest
 
@@ -32,13 +34,19 @@ def _equal(got, ws, a, b, where):
         np.testing.assert_array_equal(got[f][a:b], getattr(ws, f), err_msg="%s %s" % (where, f))
 
 
-def test_config1_warm_stepwise(oracle_mod):
+def _stepper(ctx, path):
+    return ctx.step_plain if path == "plain" else ctx.step_batched
+
+
+@pytest.mark.parametrize("path", ["plain", "batched"])
+def test_config1_warm_stepwise(oracle_mod, path):
     sc = S.config1()
     ctx = _warm_ctx(sc)
+    step_fn = _stepper(ctx, path)
     ws = _oracle(oracle_mod, sc, 0)
     carried = 0
     for step in range(240):
-        ctx.step_plain(1)
+        step_fn(1)
         ref = ws.run(1, contacts=True)
         con = ctx.contacts(0)
         assert len(con) == len(ref), step
@@ -55,11 +63,12 @@ def test_config1_warm_stepwise(oracle_mod):
     ctx.close()
 
 
+@pytest.mark.parametrize("path", ["plain", "batched"])
 @pytest.mark.parametrize("scene", ["config4", "demo"])
-def test_worlds_warm(oracle_mod, scene):
+def test_worlds_warm(oracle_mod, scene, path):
     sc = S.config4(8) if scene == "config4" else S.demo_scene()
     ctx = _warm_ctx(sc)
-    ctx.step_plain(90)
+    _stepper(ctx, path)(90)
     got = ctx.download()
     for w in range(sc.nworlds):
         ws = _oracle(oracle_mod, sc, w)
@@ -80,11 +89,46 @@ def test_set_state_empties_the_lists(oracle_mod):
     ctx.close()
 
 
-def test_batched_and_rollout_refuse_the_mode():
+def test_config3_worlds_batched_warm(oracle_mod):
+    sc = S.config3(128, nsteps=1).subset([0, 17, 40, 77, 102, 127])  # 8 to 249 bodies
+    ctx = _warm_ctx(sc)
+    for k in range(3):
+        ctx.step_batched(20)
+    got = ctx.download()
+    st = ctx.stats()
+    assert st["epa_overflow"] == 0 and st["capacity_overflow"] == 0
+    for w in range(sc.nworlds):
+        ws = _oracle(oracle_mod, sc, w)
+        ws.run(59)
+        ref = ws.run(1, contacts=True)
+        _equal(got, ws, int(sc.offsets[w]), int(sc.offsets[w + 1]), "world %d" % w)
+        con = ctx.contacts(w)
+        assert len(con) == len(ref), w
+        for f in ("index_a", "index_b", "dl_normal", "dl_tangent1", "dl_tangent2"):
+            np.testing.assert_array_equal(con[f], ref[f], err_msg="world %d %s" % (w, f))
+    ctx.close()
+
+
+def test_plain_and_batched_share_the_lists(oracle_mod):
+    sc = S.config4(8)
+    ctx = _warm_ctx(sc)
+    ctx.step_plain(30)
+    ctx.step_batched(30)   # picks up the accumulators the plain step left
+    ctx.step_plain(15)
+    ctx.step_batched(15)
+    got = ctx.download()
+    for w in range(sc.nworlds):
+        ws = _oracle(oracle_mod, sc, w)
+        ws.run(90)
+        _equal(got, ws, int(sc.offsets[w]), int(sc.offsets[w + 1]), "world %d" % w)
+    ctx.close()
+
+
+def test_autotuner_and_rollout_refuse_the_mode():
     sc = S.config4(4)
     ctx = _warm_ctx(sc)
-    with pytest.raises(runtime.NansError, match="plain step"):
-        ctx.step_batched(1)
+    with pytest.raises(runtime.NansError, match="warm"):
+        ctx.autotune(1)
     with pytest.raises(runtime.NansError, match="warm"):
         ctx.rollout(8, 4)
     ctx.set_warm_start(False)
