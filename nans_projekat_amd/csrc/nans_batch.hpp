@@ -52,9 +52,16 @@ struct BatchArgs {
     int *rec_count;
     const int *rec_cap;
     unsigned long long *cnt;
-    // opt-in warm start (k_batched_step<BLOCK, true> only): per world w, the
-    // previous and this step's contact lists (canonical pair index, three
-    // accumulators; ascending) at warm_off[w], warm_cap[w] entries each
+};
+
+// The batched step's arguments; the warm-started instantiation adds, per world w,
+// the previous and this step's contact lists (canonical pair index, three
+// accumulators; ascending) at warm_off[w], warm_cap[w] entries each.  The cold
+// step's arguments stay BatchArgs exactly.
+template <bool WARM>
+struct BatchArgsT : BatchArgs {};
+template <>
+struct BatchArgsT<true> : BatchArgs {
     const int *warm_p_prev = nullptr;
     const float *warm_s_prev = nullptr;
     const int *warm_n_prev = nullptr;

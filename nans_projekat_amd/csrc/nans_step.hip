@@ -540,7 +540,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
 // WARM: the opt-in warm start, each contact's accumulators carried over from the
 // world's previous step (a separate instantiation; the default step has none of it).
 template <int BLOCK, bool WARM = false>
-__global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgs a)
+__global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
 {
     extern __shared__ float smem[];
     constexpr int NW = BLOCK / 64;
@@ -702,7 +702,7 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgs a)
         // warm start: each contact's accumulators from the previous step's list (zero
         // when its pair was no contact there), held in rec[]'s memory, idle from here
         float *wsum = reinterpret_cast<float *>(rec);  // [3 ncon]
-        if (WARM) {
+        if constexpr (WARM) {
             const int64_t off = a.warm_off[w];
             const int *pp = a.warm_p_prev + off;
             const float *sp = a.warm_s_prev + 3 * off;
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgs a)
         solve_rounds<BLOCK, WARM>(Lb, n, con, ncon, done, headL, fl, a.k, Cf, Cr,
                                   reinterpret_cast<ContactTerms *>(epa),
                                   (int)(NW * sizeof(EpaLds) / sizeof(ContactTerms)), WARM ? wsum : nullptr);
-        if (WARM) {  // this step's list: the first warm_cap[w] contacts (contact c's sums are thread c % BLOCK's)
+        if constexpr (WARM) {  // this step's list: the first warm_cap[w] contacts (contact c's sums are thread c % BLOCK's)
             const int64_t off = a.warm_off[w];
             const int m = min(ncon, a.warm_cap[w]);
             for (int c = tid; c < m; c += BLOCK) {
@@ -1408,7 +1408,7 @@ template <int BLOCK>
 static int batched_capture(nans_ctx *ctx, int nsteps, hipStream_t cs)
 {
     nans_worlds *W = ctx->worlds;
-    BatchArgs a;
+    BatchArgsT<false> a;
     int grid = 0;
     size_t lds = 0;
     NANS_TRY(batched_prepare<BLOCK>(ctx, a, &grid, &lds));
@@ -1476,7 +1476,7 @@ template <int BLOCK>
 static int batched_warm_launches(nans_ctx *ctx, int nsteps, hipStream_t cs)
 {
     nans_worlds *W = ctx->worlds;
-    BatchArgs a;
+    BatchArgsT<true> a;
     int grid = 0;
     size_t lds = 0;
     NANS_TRY((batched_prepare<BLOCK, true>(ctx, a, &grid, &lds)));
@@ -1547,7 +1547,7 @@ template <int BLOCK>
 static int batched_recorded_launch(nans_ctx *ctx, hipStream_t s, ContactRec *rec_out, const int64_t *rec_off,
                                    int *rec_count, const int *rec_cap)
 {
-    BatchArgs a;
+    BatchArgsT<false> a;
     int grid = 0;
     size_t lds = 0;
     NANS_TRY(batched_prepare<BLOCK>(ctx, a, &grid, &lds));
