@@ -191,7 +191,9 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps);
 /* Advance every world by nsteps with the batched path: one persistent launch per
  * step over all worlds (at most 512 bodies per world).  The step (work-counter
  * reset + step kernel) is captured once in a hipGraph and replayed nsteps times;
- * the graph is kept until the batch, the parameters or the tuning change. */
+ * the graph is kept until the batch, the parameters or the tuning change.  With
+ * the warm start on (nans_set_warm_start), the warm-started step kernel is
+ * launched step by step instead. */
 int nans_step_batched(nans_ctx *ctx, int32_t nsteps);
 
 /* Device time of the last nans_step_batched call, from HIP events recorded on
