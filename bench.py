@@ -72,19 +72,46 @@ STATE_BYTES_PER_BODY_STEP = 164  # read P,V,W,A,S,m,I,F,T (92 B), write P,V,W,A 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_batched_step_config3.md")
 
 
-def measured_traffic():
-    """HBM bytes per launch of the batched step kernel: rocprofv3 FETCH_SIZE plus
-    WRITE_SIZE, each scaled by what the counter reads for a 1 GiB device copy
-    (the profile file records the commands and the factors)."""
-    if not os.path.exists(TRAFFIC_PROFILE):
+NP_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_narrowphase_config2.md")
+NP_BYTES_PER_PAIR = 108      # 2 shapes x 44 B in (type, centre, quaternion, half-extents) + 20 B out
+NP_ABI_BYTES_PER_PAIR = 144  # this ABI: 88 B in, 56 B out (flags, status, normal, depth, both points)
+
+
+def measured_traffic(path=TRAFFIC_PROFILE):
+    """HBM bytes per launch of a kernel: rocprofv3 FETCH_SIZE plus WRITE_SIZE,
+    each scaled by what the counter reads for a 1 GiB device copy (the profile
+    file records the commands and the factors)."""
+    if not os.path.exists(path):
         return None
-    with open(TRAFFIC_PROFILE) as f:
+    with open(path) as f:
         m = re.search(r"^traffic_bytes_per_launch: (\d+)$", f.read(), re.M)
     return int(m.group(1)) if m else None
 
 
 def algorithmic_flops(counts):
     return sum(FLOP_WEIGHTS[k] * counts.get(k, 0) for k in FLOP_WEIGHTS)
+
+
+COUNTS_PATH = os.path.join(ROOT, "tests", "golden", "work_counts.json")
+
+
+def load_work_counts():
+    if not os.path.exists(COUNTS_PATH):
+        return None
+    with open(COUNTS_PATH) as f:
+        return json.load(f)
+
+
+def window_flops(work, nsteps):
+    """Algorithmic flops of config-3 steps [0, nsteps) from the oracle's per-step
+    event counts; (flops, exact, candidates).  Beyond the frozen steps, the
+    run's average per step (exact = False)."""
+    per = work.get("config3_batched_per_step")
+    if per and nsteps <= len(per["candidates"]):
+        win = {k: sum(v[:nsteps]) for k, v in per.items()}
+        return algorithmic_flops(win), True, win["candidates"]
+    c3 = work["config3_batched_240_steps"]
+    return algorithmic_flops(c3) * nsteps / c3["steps"], False, None
 
 
 def _cpu_init():
@@ -104,28 +131,125 @@ def _cpu_world(args):
     return world.nbodies * nsteps, time.perf_counter() - t
 
 
-def cpu_baseline(sc, seconds, procs, nsteps):
-    """Oracle on a size-stratified sample of config-3 worlds, nsteps each, one
-    world per task on `procs` processes; stops taking worlds after `seconds`."""
+def _cpu_world_opt(args):
+    world, nsteps, opt, cull = args
+    import oracle as O
+
+    ws = O.WorldState(world, 0, "f32")
+    if cull:
+        O.counting(True, cull=True)
+    t = time.perf_counter()
+    ws.run(nsteps, opt=opt)
+    dt = time.perf_counter() - t
+    if cull:
+        O.counting(False)
+    return world.nbodies * nsteps, dt
+
+
+def _cpu_pairs(pairs):
+    import oracle as O
+    from nans_projekat_amd import scenes as S
+
+    t = time.perf_counter()
+    O.narrowphase(pairs, S.StepParams(), "f32")
+    return len(pairs["type_a"]), time.perf_counter() - t
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _stratified(sc, k):
     import numpy as np
 
     order = np.argsort(np.diff(sc.offsets))
-    sample = [int(order[i]) for i in np.linspace(0, len(order) - 1, 256).astype(int)]
-    tasks = [(sc.world(w), nsteps) for w in sample]
-    body_steps, busy = 0, 0.0
+    return [int(order[i]) for i in np.linspace(0, len(order) - 1, k).astype(int)]
+
+
+def _pool_rate(pool, fn, tasks, seconds):
+    """Units per second of the tasks on the pool's processes, taking no new task
+    after `seconds`; (units, wall, busy)."""
+    units, busy = 0, 0.0
+    t0 = time.perf_counter()
+    for u, dt in pool.imap_unordered(fn, tasks):
+        units += u
+        busy += dt
+        if time.perf_counter() - t0 > seconds:
+            break
+    return units, time.perf_counter() - t0, busy
+
+
+def cpu_baseline(sc, seconds, procs, nsteps):
+    """The binary32 oracle (the CPU restatement of the reference step, gcc -O2,
+    all pairs tested as the reference tests them) on bounded samples:
+      headline   size-stratified config-3 worlds x nsteps, one world per task on
+                 `procs` processes, for about `seconds`;
+      -O0        the same worlds with the oracle built -O0, the reference's own
+                 optimisation level (code/build.sh:5-13), for information;
+      per_config config 1 on one core (the reference is single-threaded), config 2
+                 pairs over all processes, config-4 worlds forward, and config 5
+                 on one core with the step's bounding-sphere cull (one all-pairs
+                 step without it for comparison)."""
+    from nans_projekat_amd import scenes as S
+
+    tasks = [(sc.world(w), nsteps) for w in _stratified(sc, 256)]
     with mp.get_context("spawn").Pool(procs, initializer=_cpu_init) as pool:
         pool.map(abs, range(procs))  # start every worker before the clock
-        t0 = time.perf_counter()
-        for bs, dt in pool.imap_unordered(_cpu_world, tasks):
-            body_steps += bs
-            busy += dt
-            if time.perf_counter() - t0 > seconds:
-                pool.terminate()
-                break
-    wall = time.perf_counter() - t0
-    return {"value": body_steps / wall, "unit": "world-body-steps/s", "cores": procs, "kind": "port",
-            "sample": "%d body-steps: size-stratified config-3 worlds x %d steps, all-pairs GJK/EPA "
-                      "(binary32 oracle, gcc -O2), %.1f s wall, %.1f s CPU" % (body_steps, nsteps, wall, busy)}
+        body_steps, wall, busy = _pool_rate(pool, _cpu_world, tasks, seconds)
+        pool.terminate()
+    res = {"value": body_steps / wall, "unit": "world-body-steps/s", "cores": procs, "kind": "port",
+           "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+           "sample": "%d body-steps: size-stratified config-3 worlds x %d steps, all-pairs GJK/EPA "
+                     "(binary32 oracle, gcc -O2), %.1f s wall, %.1f s CPU" % (body_steps, nsteps, wall, busy)}
+    per = {}
+    with mp.get_context("spawn").Pool(procs, initializer=_cpu_init) as pool:
+        pool.map(abs, range(procs))
+        o0 = [(sc.world(w), 24, "O0", False) for w in _stratified(sc, 64)]
+        u, w_, b_ = _pool_rate(pool, _cpu_world_opt, o0, seconds / 4)
+        res["O0"] = {"value": u / w_, "unit": "world-body-steps/s", "cores": procs,
+                     "sample": "%d body-steps: size-stratified config-3 worlds x 24 steps, oracle built -O0" % u}
+        c2 = S.config2_pairs()
+        chunks = [{k: v[lo:lo + 12500] for k, v in c2.items()} for lo in range(0, 200000, 12500)]
+        u, w_, b_ = _pool_rate(pool, _cpu_pairs, chunks, seconds)
+        per["config2"] = {"value": u / w_, "unit": "pairs/s", "cores": procs,
+                          "sample": "%d of the config-2 pairs in chunks of 12500" % u}
+        c4 = S.config4(64, nsteps=1)
+        u, w_, b_ = _pool_rate(pool, _cpu_world_opt, [(c4.world(w), 240, "O2", False) for w in range(64)],
+                               seconds / 2)
+        per["config4_forward"] = {"value": u / w_, "unit": "body-steps/s", "cores": procs,
+                                  "sample": "%d body-steps: config-4 worlds x 240 steps" % u}
+        pool.terminate()
+    _cpu_init()
+    import oracle as O
+
+    c1 = S.config1()
+    ws = O.WorldState(c1, 0, "f32")
+    t = time.perf_counter()
+    ws.run(c1.nsteps)
+    per["config1"] = {"value": c1.nbodies * c1.nsteps / (time.perf_counter() - t), "unit": "body-steps/s",
+                      "cores": 1, "sample": "config 1, %d steps" % c1.nsteps}
+    c5 = S.config5(nsteps=1)
+    for cull, steps in ((True, 2), (False, 1)):
+        ws = O.WorldState(c5, 0, "f32")
+        if cull:
+            O.counting(True, cull=True)
+        t = time.perf_counter()
+        ws.run(steps)
+        dt = time.perf_counter() - t
+        O.counting(False)
+        key = "config5_cull" if cull else "config5_all_pairs"
+        per[key] = {"value": c5.nbodies * steps / dt, "unit": "body-steps/s", "cores": 1,
+                    "sample": "config 5, first %d step(s), %s" % (steps, "bounding-sphere cull before GJK" if cull
+                                                                  else "GJK on all 33.5M pairs")}
+    res["per_config"] = per
+    return res
 
 
 def main():
@@ -195,19 +319,27 @@ def main():
                                                                           kernel_ms), file=sys.stderr)
 
     roof = None
-    counts_path = os.path.join(ROOT, "tests", "golden", "work_counts.json")
-    if os.path.exists(counts_path):
-        with open(counts_path) as f:
-            c3 = json.load(f)["config3_batched_240_steps"]
-        flops = algorithmic_flops(c3) * args.steps / c3["steps"]
+    work = load_work_counts()
+    if work is not None:
+        flops, exact, cand = window_flops(work, args.steps)
         tflops = flops / args.steps / (kernel_ms * 1e-3) / 1e12
         gbs = STATE_BYTES_PER_BODY_STEP * sc.nbodies / (kernel_ms * 1e-3) / 1e9
-        roof = {"bound": "mfma", "achieved": round(tflops, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(tflops / PEAK_FP32_TFLOPS, 5), "traffic": measured_traffic(),
-                "kernel": "k_batched_step", "compute": "vector FP32 (VALU); the kernels issue no MFMA, the "
-                "peak is MI355X's FP32 vector/matrix figure",
+        traffic = measured_traffic()
+        roof = {"bound": "valu", "achieved": round(tflops, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tflops / PEAK_FP32_TFLOPS, 5), "traffic": traffic,
+                "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch (calibrated on a device copy), "
+                                  "from %s, not measured in this run" % os.path.relpath(TRAFFIC_PROFILE, ROOT)
+                                  if traffic else None,
+                "kernel": "k_batched_step",
+                "compute": "vector FP32 (VALU): nothing on the path is a dense contraction and the kernels issue "
+                           "no MFMA; peak = MI355X FP32 vector",
                 "flops_per_launch": round(flops / args.steps), "avg_launch_ms": round(kernel_ms, 5),
-                "flops_exact_workload": args.steps == c3["steps"],
+                "flops_window": "oracle event counts of steps [0, %d) of config 3 (the timed steps), times the "
+                                "per-event weights" % args.steps if exact else
+                                "steps beyond the frozen 240: the 240-step average per step",
+                "flops_exact_workload": exact,
+                "oracle_candidates_in_window": cand,
+                "gpu_candidates_in_window": stats["candidates"],
                 "hbm": {"achieved": round(gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": round(gbs / PEAK_HBM_GBS, 6), "bytes_per_body_step": STATE_BYTES_PER_BODY_STEP}}
 
@@ -259,9 +391,28 @@ def main():
         torch.cuda.synchronize()
         nwall = time.perf_counter() - n0
         n = len(pairs["type_a"])
+        np_ms = e0.elapsed_time(e1) / args.np_reps
         out["narrowphase"] = {"value": round(n * args.np_reps / nwall, 1), "unit": "pairs/s",
-                              "kernel_ms": round(e0.elapsed_time(e1) / args.np_reps, 4),
+                              "kernel_ms": round(np_ms, 4),
                               "workload": "config2: 1M pairs (50% box-box, 30% box-sphere, 20% sphere-sphere)"}
+        if work is not None:
+            c2 = work["config2_narrowphase"]
+            fl = algorithmic_flops(c2) / c2["n_pairs"] * n
+            tf = fl / (np_ms * 1e-3) / 1e12
+            gb = NP_BYTES_PER_PAIR * n / (np_ms * 1e-3) / 1e9
+            tr = measured_traffic(NP_TRAFFIC_PROFILE)
+            out["narrowphase"]["roofline"] = {
+                "bound": "valu", "achieved": round(tf, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tf / PEAK_FP32_TFLOPS, 5), "traffic": tr,
+                "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, from %s, not measured in this "
+                                  "run" % os.path.relpath(NP_TRAFFIC_PROFILE, ROOT) if tr else None,
+                "kernel": "k_narrowphase", "flops_per_launch": round(fl),
+                "flops_per_pair": round(fl / n, 1), "avg_launch_ms": round(np_ms, 5),
+                "timing": "HIP events on the stream of the calls, around %d calls (each adds a counter reset "
+                          "and a 8-byte read-back)" % args.np_reps,
+                "hbm": {"achieved": round(gb, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(gb / PEAK_HBM_GBS, 6), "bytes_per_pair": NP_BYTES_PER_PAIR,
+                        "abi_bytes_per_pair": NP_ABI_BYTES_PER_PAIR}}
         nctx.close()
         # large single world (config 5) through the plain step: the grid broadphase
         # (the default for worlds this size) and, for comparison, the all-pairs cull

@@ -208,7 +208,9 @@ typedef struct nans_timing {
 int nans_batched_timing(nans_ctx *ctx, nans_timing *timing);
 
 /* Overwrite pos, vel, angvel, angles (and force, torque; NULL = zero) of the
- * uploaded batch in place, keeping buffers and the captured graph. */
+ * uploaded batch in place, keeping buffers and the captured graph.  The
+ * quaternions of the opt-in orientation mode are set from the given angles, as
+ * nans_upload_worlds sets them. */
 int nans_set_state(nans_ctx *ctx, const nans_bodies_soa *bodies);
 
 /* Orientation of both steps.  EULER (the default) is the reference's: Angles in
@@ -263,7 +265,8 @@ int nans_set_broadphase(nans_ctx *ctx, int32_t mode);
  * EPA polytopes are in flight (one per wave) and the width of the solver's
  * dependency rounds.  nans_autotune_batched times nsteps steps of the uploaded
  * batch with 64, 128, 256 and 512 threads per world (sizes whose LDS does not fit
- * are skipped, ms_per_step = -1), restores the state, force/torque and statistics,
+ * are skipped, ms_per_step = -1), restores the state, force/torque, quaternions
+ * and statistics,
  * and keeps the fastest size for later nans_step_batched / nans_rollout calls.
  * Results do not depend on the size.  nans_set_batched_block forces a size
  * (0: the default, 128 threads when no world holds more than 64 bodies, else 256). */

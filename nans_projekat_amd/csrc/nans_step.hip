@@ -774,8 +774,18 @@ void wfree(void *p)
     if (p) hipFree(p);
 }
 
+static void drop_graph(nans_worlds *W)
+{
+    if (W->graph) hipGraphExecDestroy(W->graph);
+    W->graph = nullptr;
+    W->graph_steps = -1;
+}
+
+// Frees the batched step's per-workgroup scratch, and with it the captured graph,
+// whose kernel arguments point into that scratch.
 static void release_batch_scratch(nans_worlds *W)
 {
+    drop_graph(W);
     wfree(W->b_cand);
     wfree(W->b_rec);
     wfree(W->b_flag);
@@ -799,13 +809,6 @@ static void release_recording(nans_worlds *W)
     W->rec_off = nullptr;
     W->rec_count = nullptr;
     W->rec_cap = nullptr;
-}
-
-static void drop_graph(nans_worlds *W)
-{
-    if (W->graph) hipGraphExecDestroy(W->graph);
-    W->graph = nullptr;
-    W->graph_steps = -1;
 }
 
 int worlds_release(nans_ctx *ctx)
@@ -1634,13 +1637,15 @@ int nans_autotune_batched(nans_ctx *ctx, int32_t nsteps, nans_tuning *out)
     hipStream_t s = ctx->stream;
     NANS_HIP(hipStreamSynchronize(s));
     const size_t N = (size_t)W->N;
-    // the trials must leave no trace: keep state, force/torque and counters
+    // the trials must leave no trace: keep state (pos, vel, angvel, angles), force,
+    // torque, the quaternions of the opt-in orientation mode and the counters
+    constexpr size_t NSTATE = F_SX, NTAIL = F_NFIELDS - F_FX;  // planes [0, F_SX) and [F_FX, F_NFIELDS)
     float *keep = nullptr;
-    NANS_TRY(dmalloc(&keep, 18 * N));
+    NANS_TRY(dmalloc(&keep, (NSTATE + NTAIL) * N));
     unsigned long long cnt[CNT_N];
     NANS_HIP(hipMemcpy(cnt, W->cnt, sizeof(cnt), hipMemcpyDeviceToHost));
-    NANS_HIP(hipMemcpyAsync(keep, W->planes, sizeof(float) * 12 * N, hipMemcpyDeviceToDevice, s));
-    NANS_HIP(hipMemcpyAsync(keep + 12 * N, W->planes + (size_t)F_FX * N, sizeof(float) * 6 * N,
+    NANS_HIP(hipMemcpyAsync(keep, W->planes, sizeof(float) * NSTATE * N, hipMemcpyDeviceToDevice, s));
+    NANS_HIP(hipMemcpyAsync(keep + NSTATE * N, W->planes + (size_t)F_FX * N, sizeof(float) * NTAIL * N,
                             hipMemcpyDeviceToDevice, s));
     const int cand[4] = {64, 128, 256, 512};
     int best = 0;
@@ -1669,8 +1674,8 @@ int nans_autotune_batched(nans_ctx *ctx, int32_t nsteps, nans_tuning *out)
             rc = r;
         }
         if (out) out->ms_per_step[c] = ms;
-        NANS_HIP(hipMemcpyAsync(W->planes, keep, sizeof(float) * 12 * N, hipMemcpyDeviceToDevice, s));
-        NANS_HIP(hipMemcpyAsync(W->planes + (size_t)F_FX * N, keep + 12 * N, sizeof(float) * 6 * N,
+        NANS_HIP(hipMemcpyAsync(W->planes, keep, sizeof(float) * NSTATE * N, hipMemcpyDeviceToDevice, s));
+        NANS_HIP(hipMemcpyAsync(W->planes + (size_t)F_FX * N, keep + NSTATE * N, sizeof(float) * NTAIL * N,
                                 hipMemcpyDeviceToDevice, s));
     }
     NANS_HIP(hipStreamSynchronize(s));
@@ -1746,6 +1751,10 @@ int nans_set_state(nans_ctx *ctx, const nans_bodies_soa *b)
     if (b->force) std::copy(b->force, b->force + 3 * N, ft.begin());
     if (b->torque) std::copy(b->torque, b->torque + 3 * N, ft.begin() + 3 * N);
     NANS_HIP(hipMemcpy(W->planes + (size_t)F_FX * N, ft.data(), sizeof(float) * ft.size(), hipMemcpyHostToDevice));
+    // the quaternions of the given angles, as at upload (used by the opt-in quaternion mode)
+    std::vector<float> q((size_t)4 * N);
+    euler_to_quat_planes(h.data() + (size_t)F_AX * N, q.data(), N);
+    NANS_HIP(hipMemcpy(W->planes + (size_t)F_QW * N, q.data(), sizeof(float) * q.size(), hipMemcpyHostToDevice));
     return warm_reset(ctx);
 }
 
