@@ -240,6 +240,8 @@ typedef struct ora_epa_rec {
     double face_sup[3][6];                 /* converged: SupA, SupB of the face's A, B, C */
     double face[ORA_TRACE_MAXF][12];       /* faces before the iteration: A, B, C, stored N */
     int32_t visible[ORA_TRACE_MAXF];       /* faces the iteration removed */
+    double dist0;                          /* face 0's distance from its stored normal (the scan's start) */
+    double dist[ORA_TRACE_MAXF];           /* each face's distance as the closest-face scan computes it */
 } ora_epa_rec;
 
 /* CheckCollision of one pair (shapes as in ora_narrowphase_*) with its trace:

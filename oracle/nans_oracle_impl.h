@@ -554,6 +554,16 @@ static double SUF(face_dist64)(const Tri *t)
     return d >= 0.0 ? d : -d;
 }
 
+/* A face's distance in ResolveCollision's closest-face scan (nans.cpp:807-822):
+ * the plane through A with the recomputed, unflipped normal, as an absolute value. */
+static REAL SUF(scan_dist)(const Tri *t)
+{
+    V3 ab = vsub(t->B.P, t->A.P);
+    V3 ac = vsub(t->C.P, t->A.P);
+    V3 nrm = vnormalize(vcross(ab, ac));
+    return gabs(vdot(nrm, t->A.P));
+}
+
 /* ResolveCollision (nans.cpp:787-904), literal: std::vector erase semantics,
  * closest face by recomputed unflipped normals, visible faces by recomputed
  * flipped normals. */
@@ -581,10 +591,7 @@ static int SUF(epa)(const Shape *a, const Shape *b, const Vert *S, const ora_np_
         REAL cur = gabs(vdot(p.t[0].N, p.t[0].A.P));
         int best = 0;
         for (int i = 0; i < p.nt; ++i) {
-            V3 ab = vsub(p.t[i].B.P, p.t[i].A.P);
-            V3 ac = vsub(p.t[i].C.P, p.t[i].A.P);
-            V3 nrm = vnormalize(vcross(ab, ac));
-            REAL dist = gabs(vdot(nrm, p.t[i].A.P));
+            REAL dist = SUF(scan_dist)(&p.t[i]);
             if (dist < cur) {
                 cur = dist;
                 best = i;
@@ -607,6 +614,8 @@ static int SUF(epa)(const Shape *a, const Shape *b, const Vert *S, const ora_np_
                 }
                 tr->best = best;
                 tr->cur = (double)cur;
+                tr->dist0 = (double)gabs(vdot(p.t[0].N, p.t[0].A.P));
+                for (int i = 0; i < p.nt; ++i) tr->dist[i] = (double)SUF(scan_dist)(&p.t[i]);
                 SUF(put3)(tr->dir, dir);
                 SUF(put3)(tr->p_new, ns.P);
                 SUF(put3)(tr->sa_new, ns.SupA);
@@ -1351,27 +1360,24 @@ static int SUF(world_run)(int32_t nb, int32_t ns,
                 SUF(constraint)(&w, &cl.c[k], p, NULL);
         } else {
             const int m = cl.n < warm->cap ? cl.n : warm->cap;
-            int64_t *np_ = (int64_t *)malloc(sizeof(int64_t) * (size_t)(m > 0 ? m : 1));
-            REAL *ns_ = (REAL *)malloc(sizeof(REAL) * 3 * (size_t)(m > 0 ? m : 1));
+            const size_t nc = (size_t)(cl.n > 0 ? cl.n : 1);
+            int64_t *pk_ = (int64_t *)malloc(sizeof(int64_t) * nc);
+            REAL *sums = (REAL *)calloc(3 * nc, sizeof(REAL));  /* each contact's accumulators */
             int j = 0;  /* both lists ascend: one merge pass finds the matches */
             for (int k = 0; k < cl.n; ++k) {
                 const Contact *c = &cl.c[k];
-                const int64_t pk = SUF(pair_index)(c->type, c->ia, c->ib, nb, ns);
-                while (j < *warm->n && warm->p[j] < pk) ++j;
-                REAL sums[3] = {0, 0, 0};
-                if (j < *warm->n && warm->p[j] == pk)
-                    for (int q = 0; q < 3; ++q) sums[q] = warm->sums[3 * j + q];
-                SUF(constraint)(&w, &cl.c[k], p, sums);
-                if (k < m) {
-                    np_[k] = pk;
-                    for (int q = 0; q < 3; ++q) ns_[3 * k + q] = sums[q];
-                }
+                pk_[k] = SUF(pair_index)(c->type, c->ia, c->ib, nb, ns);
+                while (j < *warm->n && warm->p[j] < pk_[k]) ++j;
+                if (j < *warm->n && warm->p[j] == pk_[k])
+                    for (int q = 0; q < 3; ++q) sums[3 * k + q] = warm->sums[3 * j + q];
             }
-            memcpy(warm->p, np_, sizeof(int64_t) * (size_t)m);
-            memcpy(warm->sums, ns_, sizeof(REAL) * 3 * (size_t)m);
+            for (int k = 0; k < cl.n; ++k)  /* in list order */
+                SUF(constraint)(&w, &cl.c[k], p, sums + 3 * k);
+            memcpy(warm->p, pk_, sizeof(int64_t) * (size_t)m);
+            memcpy(warm->sums, sums, sizeof(REAL) * 3 * (size_t)m);
             *warm->n = m;
-            free(np_);
-            free(ns_);
+            free(pk_);
+            free(sums);
         }
         SUF(integrate_velocities)(&w, p);
         SUF(rebuild_models)(&w);
@@ -1504,7 +1510,7 @@ int SUF(ora_frozen_record)(int32_t nb, int32_t ns,
             m->n[0] = N.x; m->n[1] = N.y; m->n[2] = N.z;
         }
         step_count[s] = cl.n;
-        for (int c = 0; c < cl.n; ++c)
+        for (int c = 0; c < cl.n; ++c)  /* SolveConstraints in list order */
             SUF(constraint)(&w, &cl.c[c], p, NULL);
         SUF(integrate_velocities)(&w, p);
         SUF(rebuild_models)(&w);
@@ -1558,8 +1564,10 @@ int SUF(ora_frozen_run)(int32_t nb, int32_t ns,
             b->F = mk3(0, 0, 0);
             b->T = mk3(0, 0, 0);
         }
-        for (int c = 0; c < step_count[s]; ++c, ++k) {
-            const ora_manifold *f = &m[k];
+        const int64_t k0 = k;
+        k += step_count[s];
+        for (int c = 0; c < step_count[s]; ++c) {  /* SolveConstraints in list order */
+            const ora_manifold *f = &m[k0 + c];
             if (f->slot_a < 0 || f->slot_a >= nb + ns || f->slot_b < -1 || f->slot_b >= nb + ns) {
                 rc = -1;
                 break;

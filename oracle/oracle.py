@@ -131,7 +131,8 @@ GJK_REC = np.dtype([("n_in", "<i4"), ("n_out", "<i4"), ("result", "<i4"), ("push
 EPA_REC = np.dtype([("nfaces", "<i4"), ("best", "<i4"), ("converged", "<i4"), ("nfaces_out", "<i4"),
                     ("cur", "<f8"), ("dir", "<f8", 3), ("p_new", "<f8", 3), ("sa_new", "<f8", 3),
                     ("sb_new", "<f8", 3), ("bary", "<f8", 3), ("face_sup", "<f8", (3, 6)),
-                    ("face", "<f8", (TRACE_MAXF, 12)), ("visible", "<i4", TRACE_MAXF)])
+                    ("face", "<f8", (TRACE_MAXF, 12)), ("visible", "<i4", TRACE_MAXF), ("dist0", "<f8"),
+                    ("dist", "<f8", TRACE_MAXF)])
 
 
 def trace_pair(shape_a: tuple, shape_b: tuple, params, precision: str = "f32", gcap: int = 80, ecap: int = 80):

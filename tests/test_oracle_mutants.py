@@ -8,7 +8,10 @@ so that the traces no longer fit).  A mistake that reverses the new faces' windi
 (PushTriangle(ns, B, A)) makes the polytope grow without bound and is left out:
 it cannot finish in a test's time.  Mistakes in Constraint's quirks (Q11-Q13:
 the subtracted body-B term, W x N, the unnormalised tangent) must fail one of the
-closed-form checks of test_constraint_quirks."""
+closed-form checks of test_constraint_quirks.  Mistakes in SolveConstraints'
+list order (reversed, or every contact reading the velocities of before the
+solve), planted in all four of the oracle's solver loops at once, must fail the
+impulse-momentum checks of test_solver_order."""
 import os
 import shutil
 import subprocess
@@ -19,6 +22,8 @@ import pytest
 import oracle as O
 import test_constraint_quirks as Q
 import test_gjk_epa_invariants as T
+import test_epa_ties as ET
+import test_solver_order as SO
 
 SRC = os.path.dirname(O.__file__)
 
@@ -61,16 +66,49 @@ CONSTRAINT_MUTANTS = {
 }
 
 
+# SolveConstraints' list order (code/nans.cpp:1538-1548), changed the same way in
+# all four solver loops of the oracle (cold step, warm step, the frozen recording
+# and the frozen replay), so that the cross-checks between those loops still
+# agree; test_solver_order's closed forms must catch them
+_LOOPS = ("for (int k = 0; k < cl.n; ++k)\n                SUF(constraint)(&w, &cl.c[k], p, NULL);",
+          "for (int k = 0; k < cl.n; ++k)  /* in list order */",
+          "for (int c = 0; c < cl.n; ++c)  /* SolveConstraints in list order */",
+          "for (int c = 0; c < step_count[s]; ++c) {  /* SolveConstraints in list order */")
+_SNAP = ("static void SUF(snap)(World *w)\n{\n"
+         "    for (int i = 0; i < w->nb; ++i) { w->box[i].V0 = w->box[i].V; w->box[i].W0 = w->box[i].W; }\n"
+         "    for (int i = 0; i < w->ns; ++i) { w->sph[i].V0 = w->sph[i].V; w->sph[i].W0 = w->sph[i].W; }\n"
+         "    w->floor.V0 = w->floor.V; w->floor.W0 = w->floor.W;\n}\n")
+SOLVER_MUTANTS = {
+    "solver_reverse_order": [
+        (_LOOPS[0], _LOOPS[0].replace("int k = 0; k < cl.n; ++k", "int k = cl.n - 1; k >= 0; --k")),
+        (_LOOPS[1], _LOOPS[1].replace("int k = 0; k < cl.n; ++k", "int k = cl.n - 1; k >= 0; --k")),
+        (_LOOPS[2], _LOOPS[2].replace("int c = 0; c < cl.n; ++c", "int c = cl.n - 1; c >= 0; --c")),
+        (_LOOPS[3], _LOOPS[3].replace("int c = 0; c < step_count[s]; ++c", "int c = step_count[s] - 1; c >= 0; --c")),
+    ],
+    # every contact reads the velocities of before the solve (a Jacobi sweep)
+    "solver_reads_presolve_velocities": [
+        ("    V3 verts[8];\n} SUF(Body);", "    V3 verts[8];\n    V3 V0, W0;\n} SUF(Body);"),
+        ("#define World SUF(World)\n", "#define World SUF(World)\n" + _SNAP),
+        ("V3 V1 = A->V, V2 = B->V, W1 = A->W, W2 = B->W;", "V3 V1 = A->V0, V2 = B->V0, W1 = A->W0, W2 = B->W0;"),
+    ] + [(loop, "SUF(snap)(&w);\n" + loop) for loop in _LOOPS],
+}
+
+
 def _build(tmp, name):
-    old, new = MUTANTS[name] if name in MUTANTS else CONSTRAINT_MUTANTS[name]
+    if name in SOLVER_MUTANTS:
+        edits = SOLVER_MUTANTS[name]
+    else:
+        edits = [MUTANTS[name] if name in MUTANTS else CONSTRAINT_MUTANTS[name]]
     d = os.path.join(str(tmp), name)
     os.makedirs(d)
     for f in ("nans_oracle.h", "nans_oracle_impl.h", "nans_oracle_f32.c", "nans_oracle_f64.c"):
         shutil.copy(os.path.join(SRC, f), d)
     p = os.path.join(d, "nans_oracle_impl.h")
     s = open(p).read()
-    assert s.count(old) == 1, name
-    open(p, "w").write(s.replace(old, new))
+    for old, new in edits:
+        assert s.count(old) == 1, (name, old)
+        s = s.replace(old, new)
+    open(p, "w").write(s)
     lib = os.path.join(d, "libnans_oracle_mutant.so")
     subprocess.run(["gcc", "-std=c11", "-fPIC", "-O2", "-ffp-contract=off", "-fno-fast-math", "-msse2",
                     "-mfpmath=sse", "-shared", "-o", lib, os.path.join(d, "nans_oracle_f32.c"),
@@ -120,3 +158,27 @@ def test_planted_constraint_mistake_is_caught(tmp_path, monkeypatch, name):
         except AssertionError:
             failed += 1
     assert failed >= 1, name
+
+
+@pytest.mark.parametrize("name", sorted(SOLVER_MUTANTS))
+def test_planted_solver_order_mistake_is_caught(tmp_path, monkeypatch, name):
+    """The mutant keeps the oracle's loops consistent with each other (the frozen
+    replay still equals the step), yet the closed forms of test_solver_order fail."""
+    monkeypatch.setattr(O, "LIB_PATH", _build(tmp_path, name))
+    failed = 0
+    for check in (SO.test_box_then_sphere_serial_order, SO.test_stack_serial_order):
+        try:
+            check(O)
+        except AssertionError:
+            failed += 1
+    assert failed == 2, name
+
+
+def test_planted_tie_rule_mistake_is_caught(tmp_path, monkeypatch):
+    """A '<=' closest-face scan (ties to the last face) fails test_epa_ties."""
+    name = "epa_ties_to_last_face"
+    MUTANTS_TIES = {name: ("if (dist < cur) {", "if (dist <= cur) {")}
+    monkeypatch.setitem(MUTANTS, name, MUTANTS_TIES[name])
+    monkeypatch.setattr(O, "LIB_PATH", _build(tmp_path, name))
+    with pytest.raises(AssertionError):
+        ET.test_closest_face_ties_go_to_the_first_face(O)
