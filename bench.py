@@ -252,6 +252,116 @@ def cpu_baseline(sc, seconds, procs, nsteps):
     return res
 
 
+def narrowphase_leg(args, work):
+    """Config 2: 1M pairs through nans_narrowphase, args.np_reps timed calls."""
+    import torch
+
+    from nans_projekat_amd import runtime
+    from nans_projekat_amd import scenes as S
+
+    pairs = S.config2_pairs()
+    nctx = runtime.Context(S.StepParams())
+    dp = nctx.upload_pairs(pairs)
+    o = nctx.narrowphase(dp)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    n0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.np_reps):
+        nctx.narrowphase(dp, o)
+    e1.record()
+    torch.cuda.synchronize()
+    nwall = time.perf_counter() - n0
+    n = len(pairs["type_a"])
+    np_ms = e0.elapsed_time(e1) / args.np_reps
+    res = {"value": round(n * args.np_reps / nwall, 1), "unit": "pairs/s", "kernel_ms": round(np_ms, 4),
+           "workload": "config2: 1M pairs (50% box-box, 30% box-sphere, 20% sphere-sphere)"}
+    if work is not None:
+        c2 = work["config2_narrowphase"]
+        fl = algorithmic_flops(c2) / c2["n_pairs"] * n
+        tf = fl / (np_ms * 1e-3) / 1e12
+        gb = NP_BYTES_PER_PAIR * n / (np_ms * 1e-3) / 1e9
+        tr = measured_traffic(NP_TRAFFIC_PROFILE)
+        res["roofline"] = {
+            "bound": "valu", "achieved": round(tf, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tf / PEAK_FP32_TFLOPS, 5), "traffic": tr,
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, from %s, not measured in this "
+                              "run" % os.path.relpath(NP_TRAFFIC_PROFILE, ROOT) if tr else None,
+            "kernel": "k_narrowphase", "flops_per_launch": round(fl),
+            "flops_per_pair": round(fl / n, 1), "avg_launch_ms": round(np_ms, 5),
+            "timing": "HIP events on the stream of the calls, around %d calls (each adds a counter reset "
+                      "and a 8-byte read-back)" % args.np_reps,
+            "hbm": {"achieved": round(gb, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": round(gb / PEAK_HBM_GBS, 6), "bytes_per_pair": NP_BYTES_PER_PAIR,
+                    "abi_bytes_per_pair": NP_ABI_BYTES_PER_PAIR}}
+    nctx.close()
+    return res
+
+
+def large_world_leg(args):
+    """Config 5 through the plain step, grid broadphase and all-pairs cull."""
+    import torch
+
+    from nans_projekat_amd import runtime
+    from nans_projekat_amd import scenes as S
+
+    # the grid broadphase is the default for worlds this size
+    big = S.config5()
+    lctx = runtime.Context(big.params)
+    lwall = {}
+    for mode in ("all_pairs", "grid"):
+        lctx.upload(big)
+        lctx.set_broadphase(mode)
+        torch.cuda.synchronize()
+        l0 = time.perf_counter()
+        lctx.step_plain(args.large_steps)
+        torch.cuda.synchronize()
+        lwall[mode] = time.perf_counter() - l0
+    lst = lctx.stats()
+    res = {"value": round(big.nbodies * args.large_steps / lwall["grid"], 1), "unit": "body-steps/s",
+           "ms_per_step": round(1e3 * lwall["grid"] / args.large_steps, 4),
+           "workload": "config5: 8192 boxes, dt 1/240, 20 iterations, %d steps, plain step, grid broadphase"
+                       % args.large_steps,
+           "all_pairs_ms_per_step": round(1e3 * lwall["all_pairs"] / args.large_steps, 4),
+           "canonical_pairs_per_s": round(lst["pair_tests"] / lwall["grid"], 1),
+           "candidates": lst["candidates"], "contacts": lst["contacts"]}
+    lctx.close()
+    return res
+
+
+def differentiable_leg(args):
+    """Config 4: checkpointed rollout plus the hand-written adjoint."""
+    import numpy as np
+    import torch
+
+    from nans_projekat_amd import runtime
+    from nans_projekat_amd import scenes as S
+
+    dif = S.config4()
+    dctx = runtime.Context(dif.params)
+    dctx.upload(dif)
+    dl = np.full((dif.nbodies, 3), 1e-3, np.float32)
+    dctx.rollout(8, 4)
+    dctx.adjoint(dl_pos=dl)  # warm-up: graph capture, scratch allocation
+    dctx.set_state(dif)
+    torch.cuda.synchronize()
+    d0 = time.perf_counter()
+    dctx.rollout(dif.nsteps, 16)
+    torch.cuda.synchronize()
+    d1 = time.perf_counter()
+    dctx.adjoint(dl_pos=dl)
+    torch.cuda.synchronize()
+    d2 = time.perf_counter()
+    bsteps = dif.nbodies * dif.nsteps
+    res = {"value": round(bsteps / (d2 - d0), 1), "unit": "body-steps/s (forward + adjoint)",
+           "forward_ms_per_step": round(1e3 * (d1 - d0) / dif.nsteps, 4),
+           "adjoint_ms_per_step": round(1e3 * (d2 - d1) / dif.nsteps, 4),
+           "workload": "config4: 1024 worlds x 16 cubes, %d steps, state checkpointed every 16 steps; gradients "
+                       "w.r.t. mass, moi, friction, restitution and the initial state" % dif.nsteps}
+    dctx.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,6 +376,8 @@ def main():
     ap.add_argument("--cpu-procs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--headline-only", action="store_true", help="only the batched config-3 measurement")
+    ap.add_argument("--leg", choices=("narrowphase", "large_world", "differentiable"),
+                    help="run only this secondary measurement and print its JSON (for profiling)")
     ap.add_argument("--large-steps", type=int, default=480, help="config-5 steps of the large-world leg")
     ap.add_argument("--mode-steps", type=int, default=60, help="steps of each opt-in mode measurement")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -286,6 +398,13 @@ def main():
         print("bench.py measures one GPU: this build runs no multi-GPU path (requested --gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+
+    if args.leg:
+        leg = {"narrowphase": lambda: narrowphase_leg(args, load_work_counts()),
+               "large_world": lambda: large_world_leg(args),
+               "differentiable": lambda: differentiable_leg(args)}[args.leg]
+        print(json.dumps({args.leg: leg()}))
+        return
 
     import numpy as np
     import torch
@@ -376,92 +495,9 @@ def main():
         out["plain_gpu"] = {"value": round(plain, 1), "unit": "world-body-steps/s",
                             "sample": "%d size-stratified config-3 worlds x %d steps" % (len(pick), args.plain_steps)}
         out["batched_vs_plain"] = round(value / plain, 2)
-        # narrowphase microbench (config 2)
-        pairs = S.config2_pairs()
-        nctx = runtime.Context(S.StepParams())
-        dp = nctx.upload_pairs(pairs)
-        o = nctx.narrowphase(dp)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        n0 = time.perf_counter()
-        e0.record()
-        for _ in range(args.np_reps):
-            nctx.narrowphase(dp, o)
-        e1.record()
-        torch.cuda.synchronize()
-        nwall = time.perf_counter() - n0
-        n = len(pairs["type_a"])
-        np_ms = e0.elapsed_time(e1) / args.np_reps
-        out["narrowphase"] = {"value": round(n * args.np_reps / nwall, 1), "unit": "pairs/s",
-                              "kernel_ms": round(np_ms, 4),
-                              "workload": "config2: 1M pairs (50% box-box, 30% box-sphere, 20% sphere-sphere)"}
-        if work is not None:
-            c2 = work["config2_narrowphase"]
-            fl = algorithmic_flops(c2) / c2["n_pairs"] * n
-            tf = fl / (np_ms * 1e-3) / 1e12
-            gb = NP_BYTES_PER_PAIR * n / (np_ms * 1e-3) / 1e9
-            tr = measured_traffic(NP_TRAFFIC_PROFILE)
-            out["narrowphase"]["roofline"] = {
-                "bound": "valu", "achieved": round(tf, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(tf / PEAK_FP32_TFLOPS, 5), "traffic": tr,
-                "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, from %s, not measured in this "
-                                  "run" % os.path.relpath(NP_TRAFFIC_PROFILE, ROOT) if tr else None,
-                "kernel": "k_narrowphase", "flops_per_launch": round(fl),
-                "flops_per_pair": round(fl / n, 1), "avg_launch_ms": round(np_ms, 5),
-                "timing": "HIP events on the stream of the calls, around %d calls (each adds a counter reset "
-                          "and a 8-byte read-back)" % args.np_reps,
-                "hbm": {"achieved": round(gb, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                        "frac": round(gb / PEAK_HBM_GBS, 6), "bytes_per_pair": NP_BYTES_PER_PAIR,
-                        "abi_bytes_per_pair": NP_ABI_BYTES_PER_PAIR}}
-        nctx.close()
-        # large single world (config 5) through the plain step: the grid broadphase
-        # (the default for worlds this size) and, for comparison, the all-pairs cull
-        big = S.config5()
-        lctx = runtime.Context(big.params)
-        lwall = {}
-        for mode in ("all_pairs", "grid"):
-            lctx.upload(big)
-            lctx.set_broadphase(mode)
-            torch.cuda.synchronize()
-            l0 = time.perf_counter()
-            lctx.step_plain(args.large_steps)
-            torch.cuda.synchronize()
-            lwall[mode] = time.perf_counter() - l0
-        lst = lctx.stats()
-        out["large_world"] = {"value": round(big.nbodies * args.large_steps / lwall["grid"], 1),
-                              "unit": "body-steps/s",
-                              "ms_per_step": round(1e3 * lwall["grid"] / args.large_steps, 4),
-                              "workload": "config5: 8192 boxes, dt 1/240, 20 iterations, %d steps, plain step, "
-                                          "grid broadphase" % args.large_steps,
-                              "all_pairs_ms_per_step": round(1e3 * lwall["all_pairs"] / args.large_steps, 4),
-                              "canonical_pairs_per_s": round(lst["pair_tests"] / lwall["grid"], 1),
-                              "candidates": lst["candidates"],
-                              "contacts": lst["contacts"]}
-        lctx.close()
-        # differentiable rollout (config 4): checkpointed forward + hand-written adjoint
-        dif = S.config4()
-        dctx = runtime.Context(dif.params)
-        dctx.upload(dif)
-        dl = np.full((dif.nbodies, 3), 1e-3, np.float32)
-        dctx.rollout(8, 4)
-        dctx.adjoint(dl_pos=dl)  # warm-up: graph capture, scratch allocation
-        dctx.set_state(dif)
-        torch.cuda.synchronize()
-        d0 = time.perf_counter()
-        dctx.rollout(dif.nsteps, 16)
-        torch.cuda.synchronize()
-        d1 = time.perf_counter()
-        dctx.adjoint(dl_pos=dl)
-        torch.cuda.synchronize()
-        d2 = time.perf_counter()
-        bsteps = dif.nbodies * dif.nsteps
-        out["differentiable"] = {"value": round(bsteps / (d2 - d0), 1), "unit": "body-steps/s (forward + adjoint)",
-                                 "forward_ms_per_step": round(1e3 * (d1 - d0) / dif.nsteps, 4),
-                                 "adjoint_ms_per_step": round(1e3 * (d2 - d1) / dif.nsteps, 4),
-                                 "workload": "config4: 1024 worlds x 16 cubes, %d steps, state checkpointed every "
-                                             "16 steps; gradients w.r.t. mass, moi, friction, restitution and the "
-                                             "initial state" % dif.nsteps}
-        dctx.close()
+        out["narrowphase"] = narrowphase_leg(args, work)
+        out["large_world"] = large_world_leg(args)
+        out["differentiable"] = differentiable_leg(args)
         # opt-in modes: quaternion orientation on the batched config-3 step, warm start on
         # the config-5 plain step (each against the reference's mode on the same steps)
         modes = {}

@@ -10,10 +10,11 @@
 //   length(v)    = sqrt(dot(v,v))
 //   abs(x)       = x >= 0 ? x : -x
 //   mat4 * vec4  = (c0*x + c1*y) + (c2*z + c3*w)
-// sin/cos of the Model rebuild are the binary64 values rounded to binary32
-// (the reference calls the binary32 libm functions; both round the same
-// mathematical value and agree except when it lies within ~1e-16 relative of
-// a binary32 rounding boundary).
+// sin/cos of the Model rebuild are the correctly rounded binary32 values (the
+// binary64 results rounded once).  The reference calls the binary32 sinf/cosf
+// of whatever libm it is linked with; glibc's are not correctly rounded and
+// differ from these by one ulp for about 1% of the angles the steps produce,
+// so results equal a glibc-linked reference's only where no such angle occurs.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -26,15 +26,17 @@
  *   mat4 * vec4   = (c0*x + c1*y) + (c2*z + c3*w)
  *   translate, rotate (axis-angle), scale, radians as in matrix_transform.inl.
  * The binary32 sin/cos of rotate are the correctly rounded values (the binary64
- * results rounded to binary32); the reference's libm sinf/cosf agree with them
- * except for about 1% of arguments, where they are one ulp off.
+ * results rounded to binary32); glibc's sinf/cosf agree with them except for
+ * about 1% of arguments, where they are one ulp off.  Built with ORA_LIBM_SINF
+ * the oracle calls the host libm instead (lib/libnans_oracle_libm.so), to
+ * measure what that choice moves.
  *
  * EvolveSimplex and ResolveCollision are checked call by call through
  * ora_trace_pair_* against properties the geometry fixes (tests/
- * test_gjk_epa_invariants.py, with planted mistakes in tests/
- * test_oracle_mutants.py).  Parity unpinned: exact ties between EPA face
- * distances (resolved by the first index, the strict '<' scan) are pinned only
- * by this restatement.
+ * test_gjk_epa_invariants.py), the closest-face tie rule on exact distance ties
+ * (tests/test_epa_ties.py) and SolveConstraints' list order by impulse-momentum
+ * algebra (tests/test_solver_order.py), each with planted mistakes in tests/
+ * test_oracle_mutants.py.
  */
 #ifndef NANS_ORACLE_H
 #define NANS_ORACLE_H

@@ -23,9 +23,16 @@
  * through matrix_transform.inl).  Those are implementation-defined to within an ulp
  * (glibc's sinf/cosf are not correctly rounded for about 1% of arguments), so the
  * restatement uses the correctly rounded binary32 value, obtained by rounding the
- * binary64 result. */
+ * binary64 result.  Built with ORA_LIBM_SINF (libnans_oracle_libm.so) it calls the
+ * host libm's sinf/cosf instead, as a glibc-linked build of the reference does;
+ * that variant measures how far the choice moves the results. */
+#ifdef ORA_LIBM_SINF
+static inline float ora_sinf(float x) { return sinf(x); }
+static inline float ora_cosf(float x) { return cosf(x); }
+#else
 static inline float ora_sinf(float x) { return (float)sin((double)x); }
 static inline float ora_cosf(float x) { return (float)cos((double)x); }
+#endif
 #define SIN ora_sinf
 #define COS ora_cosf
 #else

@@ -15,6 +15,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libnans_oracle.so")
 LIB_O0_PATH = os.path.join(HERE, "lib", "libnans_oracle_O0.so")
+# the binary32 oracle with the Model rebuild's sin/cos from the host libm's sinf/cosf
+LIB_LIBM_PATH = os.path.join(HERE, "lib", "libnans_oracle_libm.so")
 
 NSTAT = 8
 STAT_GJK_CALLS, STAT_GJK_RESULT, STAT_EPA_ITERS, STAT_EPA_RESULT = 0, 1, 2, 3
@@ -68,7 +70,7 @@ def build() -> None:
 
 
 def lib(opt: str = "O2"):
-    path = LIB_PATH if opt == "O2" else LIB_O0_PATH
+    path = {"O2": LIB_PATH, "O0": LIB_O0_PATH, "libm": LIB_LIBM_PATH}[opt]
     if path not in _libs:
         if not os.path.exists(path):
             build()

@@ -71,6 +71,24 @@ def test_edge_cases_bit_exact(gpu_ctx, oracle_mod, islands):
     _assert_bit_exact(out, ref, oracle_mod)
 
 
+@pytest.mark.parametrize("islands", [0, 1])
+def test_exact_distance_ties_bit_exact(gpu_ctx, oracle_mod, islands):
+    """Axis-aligned box-box overlaps whose EPA polytopes hold faces at bit-identical
+    distances (test_epa_ties pins the oracle's first-face rule on them): the
+    wave-level closest-face reductions pick the same faces."""
+    from test_epa_ties import tie_cases
+
+    rows = tie_cases()
+    n = len(rows)
+    col = lambda side, i, w: np.array([r[side][i] for r in rows], np.float32).reshape(n, w)
+    pairs = dict(type_a=np.array([r[0][0] for r in rows], np.int32), pos_a=col(0, 1, 3), quat_a=col(0, 2, 4),
+                 half_a=col(0, 3, 3), type_b=np.array([r[1][0] for r in rows], np.int32), pos_b=col(1, 1, 3),
+                 quat_b=col(1, 2, 4), half_b=col(1, 3, 3))
+    out, ref = _run_both(gpu_ctx, oracle_mod, pairs, islands)
+    _assert_bit_exact(out, ref, oracle_mod)
+    assert out["contact"].sum() >= n // 2
+
+
 def test_empty_input(gpu_ctx):
     pairs = {k: v[:0] for k, v in S.config2_pairs(4).items()}
     out = gpu_ctx.narrowphase(gpu_ctx.upload_pairs(pairs))
