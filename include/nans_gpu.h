@@ -134,6 +134,19 @@ int nans_set_stream(nans_ctx *ctx, void *hip_stream);
 int nans_narrowphase(nans_ctx *ctx, int64_t n, const nans_shapes_soa *a, const nans_shapes_soa *b,
                      const nans_narrow_out *out, int64_t *n_overflow);
 
+/* nans_narrowphase without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors as nans_narrowphase, except that at most
+ * NANS_NARROWPHASE_NOALLOC_MAX_PAIRS pairs are accepted per call (more:
+ * NANS_ERR_CAPACITY; split the call).  Its EPA queue lives in a static device
+ * array of 25 MiB per device, held until the process ends and shared by every
+ * call on that device: calls that overlap on one device (several threads or
+ * streams at once) race on it. */
+#define NANS_NARROWPHASE_NOALLOC_MAX_PAIRS 262144
+int nans_narrowphase_noalloc(nans_ctx *ctx, int64_t n, const nans_shapes_soa *a, const nans_shapes_soa *b,
+                             const nans_narrow_out *out, int64_t *n_overflow);
+
 /* ------------------------------------------------------------------ worlds */
 
 /* Bodies of a batch of worlds, host pointers.  World w owns bodies

@@ -84,12 +84,6 @@ inline int dmalloc(T **p, size_t n)
     return NANS_OK;
 }
 
-#define NANS_TRY(x)             \
-    do {                        \
-        int rc_ = (x);          \
-        if (rc_) return rc_;    \
-    } while (0)
-
 void wfree(void *p);
 StepConsts step_consts(const nans_params &p, int orientation);
 int warm_reset(nans_ctx *ctx);

@@ -2,6 +2,7 @@
 // C ABI declared in include/nans_gpu.h.
 #include "nans_common.hpp"
 
+#include <mutex>
 #include <new>
 
 namespace nans {
@@ -106,6 +107,39 @@ __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
     if (i < n) p[i] = v;
 }
 
+// Scratch of nans_narrowphase_noalloc, one copy per device for the life of the
+// process: the EPA overflow counter (in its own 256-byte slot), then the EPA queue
+// of k_narrowphase -- a candidate index and a GJK simplex per pair -- for up to
+// NANS_NARROWPHASE_NOALLOC_MAX_PAIRS pairs (25 MiB in all).
+constexpr size_t NA_CNT_BYTES = 256;
+constexpr size_t NA_QIDX_BYTES = sizeof(int) * (size_t)NANS_NARROWPHASE_NOALLOC_MAX_PAIRS;
+constexpr size_t NA_QSIM_BYTES = sizeof(SimplexRec) * (size_t)NANS_NARROWPHASE_NOALLOC_MAX_PAIRS;
+constexpr size_t NA_BYTES = NA_CNT_BYTES + NA_QIDX_BYTES + NA_QSIM_BYTES;
+static_assert(NA_QIDX_BYTES % 256 == 0, "the simplex records start 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// Device address of each device's copy of scratch[], looked up once per device
+// (with that device current) by nans_create.  The name scratch is a host-side
+// stand-in in host code; only these addresses are passed to kernels.
+constexpr int NA_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[NA_MAX_DEVICES];
+
+static int noalloc_scratch_init(int device, void **out)
+{
+    NANS_REQUIRE(device >= 0 && device < NA_MAX_DEVICES, NANS_ERR_INVALID, "device %d beyond %d", device,
+                 NA_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *ptr = nullptr;
+        NANS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        NANS_REQUIRE(ptr, NANS_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_dev[device] = ptr;
+    }
+    *out = g_scratch_dev[device];
+    return NANS_OK;
+}
+
 }  // namespace nans
 
 using namespace nans;
@@ -175,6 +209,12 @@ int nans_create(nans_ctx **out, const nans_params *p)
     c->np_cap = 0;
     c->np_qidx = nullptr;
     c->np_qsim = nullptr;
+    c->na_scratch = nullptr;
+    rc = noalloc_scratch_init(q.device, &c->na_scratch);
+    if (rc) {
+        delete c;
+        return rc;
+    }
     hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -232,8 +272,12 @@ int nans_set_broadphase(nans_ctx *ctx, int32_t mode)
     return NANS_OK;
 }
 
-int nans_narrowphase(nans_ctx *ctx, int64_t n, const nans_shapes_soa *a, const nans_shapes_soa *b,
-                     const nans_narrow_out *out, int64_t *n_overflow)
+}  // extern "C"
+
+namespace nans {
+
+static int narrow_check(const nans_ctx *ctx, int64_t n, const nans_shapes_soa *a, const nans_shapes_soa *b,
+                        const nans_narrow_out *out)
 {
     NANS_REQUIRE(ctx, NANS_ERR_INVALID, "ctx is NULL");
     NANS_REQUIRE(n >= 0, NANS_ERR_INVALID, "n < 0");
@@ -241,34 +285,68 @@ int nans_narrowphase(nans_ctx *ctx, int64_t n, const nans_shapes_soa *a, const n
     NANS_REQUIRE(a && b && out, NANS_ERR_INVALID, "shape or output descriptor is NULL");
     NANS_REQUIRE(n == 0 || (a->type && a->pos && a->quat && a->half && b->type && b->pos && b->quat && b->half),
                  NANS_ERR_INVALID, "a shape array is NULL");
-    NANS_HIP(hipSetDevice(ctx->p.device));
+    return NANS_OK;
+}
+
+// k_narrowphase over n pairs on the context's stream with the given overflow
+// counter and EPA queue (n entries), then the counter to the host.
+static int narrow_launch(nans_ctx *ctx, int64_t n, const nans_shapes_soa *a, const nans_shapes_soa *b,
+                         const nans_narrow_out *out, unsigned long long *cnt, int *qidx, SimplexRec *qsim,
+                         int64_t *n_overflow)
+{
     hipStream_t s = ctx->stream;
-    k_fill_u64<<<1, 64, 0, s>>>(ctx->d_counters, 1, 0ull);
+    k_fill_u64<<<1, 64, 0, s>>>(cnt, 1, 0ull);
     NANS_HIP(hipGetLastError());
     if (n > 0) {
-        if (n > ctx->np_cap) {
-            hipFree(ctx->np_qidx);
-            hipFree(ctx->np_qsim);
-            ctx->np_qidx = nullptr;
-            ctx->np_qsim = nullptr;
-            ctx->np_cap = 0;
-            NANS_HIP(hipMalloc((void **)&ctx->np_qidx, sizeof(int) * (size_t)n));
-            NANS_HIP(hipMalloc((void **)&ctx->np_qsim, sizeof(SimplexRec) * (size_t)n));
-            ctx->np_cap = n;
-        }
         NarrowIn ia{a->type, a->pos, a->quat, a->half};
         NarrowIn ib{b->type, b->pos, b->quat, b->half};
         NarrowOutDev o{out->gjk, out->contact, out->status, out->normal, out->point_a, out->point_b, out->depth};
         unsigned grid = (unsigned)((n + NP_RANGE - 1) / NP_RANGE);
-        k_narrowphase<<<grid, 256, 0, s>>>(n, ia, ib, narrow_params(ctx->p), o, ctx->d_counters, ctx->np_qidx,
-                                           (SimplexRec *)ctx->np_qsim);
+        k_narrowphase<<<grid, 256, 0, s>>>(n, ia, ib, narrow_params(ctx->p), o, cnt, qidx, qsim);
         NANS_HIP(hipGetLastError());
     }
     unsigned long long ov = 0;
-    NANS_HIP(hipMemcpyAsync(&ov, ctx->d_counters, sizeof(ov), hipMemcpyDeviceToHost, s));
+    NANS_HIP(hipMemcpyAsync(&ov, cnt, sizeof(ov), hipMemcpyDeviceToHost, s));
     NANS_HIP(hipStreamSynchronize(s));
     if (n_overflow) *n_overflow = (int64_t)ov;
     return NANS_OK;
+}
+
+}  // namespace nans
+
+extern "C" {
+
+int nans_narrowphase(nans_ctx *ctx, int64_t n, const nans_shapes_soa *a, const nans_shapes_soa *b,
+                     const nans_narrow_out *out, int64_t *n_overflow)
+{
+    NANS_TRY(narrow_check(ctx, n, a, b, out));
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    if (n > ctx->np_cap) {
+        hipFree(ctx->np_qidx);
+        hipFree(ctx->np_qsim);
+        ctx->np_qidx = nullptr;
+        ctx->np_qsim = nullptr;
+        ctx->np_cap = 0;
+        NANS_HIP(hipMalloc((void **)&ctx->np_qidx, sizeof(int) * (size_t)n));
+        NANS_HIP(hipMalloc((void **)&ctx->np_qsim, sizeof(SimplexRec) * (size_t)n));
+        ctx->np_cap = n;
+    }
+    return narrow_launch(ctx, n, a, b, out, ctx->d_counters, ctx->np_qidx, (SimplexRec *)ctx->np_qsim, n_overflow);
+}
+
+int nans_narrowphase_noalloc(nans_ctx *ctx, int64_t n, const nans_shapes_soa *a, const nans_shapes_soa *b,
+                             const nans_narrow_out *out, int64_t *n_overflow)
+{
+    NANS_TRY(narrow_check(ctx, n, a, b, out));
+    NANS_REQUIRE(n <= NANS_NARROWPHASE_NOALLOC_MAX_PAIRS, NANS_ERR_CAPACITY,
+                 "n = %lld exceeds the %d pairs of the no-allocation scratch; split the call",
+                 (long long)n, NANS_NARROWPHASE_NOALLOC_MAX_PAIRS);
+    NANS_REQUIRE(ctx->na_scratch, NANS_ERR_STATE, "the context has no scratch address");
+    NANS_HIP(hipSetDevice(ctx->p.device));
+    unsigned char *base = static_cast<unsigned char *>(ctx->na_scratch);
+    return narrow_launch(ctx, n, a, b, out, reinterpret_cast<unsigned long long *>(base),
+                         reinterpret_cast<int *>(base + NA_CNT_BYTES),
+                         reinterpret_cast<SimplexRec *>(base + NA_CNT_BYTES + NA_QIDX_BYTES), n_overflow);
 }
 
 }  // extern "C"

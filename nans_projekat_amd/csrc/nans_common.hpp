@@ -34,6 +34,12 @@ void set_error(const char *fmt, ...);
         }                               \
     } while (0)
 
+#define NANS_TRY(x)             \
+    do {                        \
+        int rc_ = (x);          \
+        if (rc_) return rc_;    \
+    } while (0)
+
 inline NarrowParams narrow_params(const nans_params &p)
 {
     NarrowParams q;
@@ -61,6 +67,7 @@ struct nans_ctx {
     int64_t np_cap;                  // narrowphase entry point: queue capacity (pairs)
     int *np_qidx;                    // queued GJK hits
     void *np_qsim;                   // their simplices (SimplexRec)
+    void *na_scratch;                // this device's scratch of nans_narrowphase_noalloc
     int broadphase;                  // NANS_BROADPHASE_* of the plain step
     int orientation;                 // NANS_ORIENT_* of both steps
     int warm;                        // opt-in warm start of the plain and batched steps

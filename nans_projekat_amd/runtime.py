@@ -25,6 +25,7 @@ SHAPE_BOX, SHAPE_SPHERE = 0, 1
 CC, CS, CF, SS, SF = 0, 1, 2, 3, 4
 GJK_NO, GJK_FOUND, GJK_CAP = 0, 1, 2
 EPA_NONE, EPA_CONTACT, EPA_BARY, EPA_NONFINITE, EPA_ITER, EPA_EMPTY, EPA_OVERFLOW = range(7)
+NARROWPHASE_NOALLOC_MAX_PAIRS = 262144
 
 
 class NansError(RuntimeError):
@@ -193,8 +194,10 @@ class Context:
                     point_b=torch.empty((3, n), dtype=torch.float32, device=dev),
                     status=torch.empty(n, dtype=torch.int32, device=dev))
 
-    def narrowphase(self, dpairs: dict, out: dict | None = None) -> dict:
-        """CheckCollision (code/nans.cpp:906-966) on device-resident pairs."""
+    def narrowphase(self, dpairs: dict, out: dict | None = None, noalloc: bool = False) -> dict:
+        """CheckCollision (code/nans.cpp:906-966) on device-resident pairs.
+        noalloc: through nans_narrowphase_noalloc (no device allocation; at most
+        NARROWPHASE_NOALLOC_MAX_PAIRS pairs per call)."""
         n = int(dpairs["n"])
         if out is None:
             out = self.alloc_narrow_out(n)
@@ -203,8 +206,8 @@ class Context:
         o = NarrowOut(*[_ptr(out[k]) for k in ("gjk", "contact", "normal", "depth", "point_a", "point_b", "status")])
         nov = ctypes.c_int64(0)
         self._sync_stream()
-        check(self.lib.nans_narrowphase(self.h, ctypes.c_int64(n), ctypes.byref(a), ctypes.byref(b),
-                                        ctypes.byref(o), ctypes.byref(nov)))
+        fn = self.lib.nans_narrowphase_noalloc if noalloc else self.lib.nans_narrowphase
+        check(fn(self.h, ctypes.c_int64(n), ctypes.byref(a), ctypes.byref(b), ctypes.byref(o), ctypes.byref(nov)))
         out["n_overflow"] = nov.value
         return out
 
