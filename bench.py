@@ -32,6 +32,7 @@ Also reported on the same line:
 Usage: python bench.py [--gpus 1] [--steps 240] [--warmup 5]
 """
 import argparse
+import dataclasses
 import json
 import multiprocessing as mp
 import os
@@ -274,7 +275,18 @@ def narrowphase_leg(args, work):
     nwall = time.perf_counter() - n0
     n = len(pairs["type_a"])
     np_ms = e0.elapsed_time(e1) / args.np_reps
+    # the FP64-islands mode on the same pairs
+    nctx.set_params(S.StepParams(fp64_islands=1))
+    nctx.narrowphase(dp, o)
+    torch.cuda.synchronize()
+    i0, i1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    i0.record()
+    for _ in range(args.np_reps):
+        nctx.narrowphase(dp, o)
+    i1.record()
+    torch.cuda.synchronize()
     res = {"value": round(n * args.np_reps / nwall, 1), "unit": "pairs/s", "kernel_ms": round(np_ms, 4),
+           "fp64_islands_kernel_ms": round(i0.elapsed_time(i1) / args.np_reps, 4),
            "workload": "config2: 1M pairs (50% box-box, 30% box-sphere, 20% sphere-sphere)"}
     if work is not None:
         c2 = work["config2_narrowphase"]
@@ -516,7 +528,15 @@ def main():
         qctx.set_state(sc)  # empties the warm-start lists
         qctx.step_batched(args.mode_steps)
         modes["batched_warm"] = qctx.batched_timing()["elapsed_ms"] / args.mode_steps
+        # the FP64-islands mode on the same batched steps
+        qctx.set_params(dataclasses.replace(sc.params, fp64_islands=1))
+        qctx.set_state(sc)
+        qctx.step_batched(2)
+        qctx.set_state(sc)
+        qctx.step_batched(args.mode_steps)
+        modes["islands"] = qctx.batched_timing()["elapsed_ms"] / args.mode_steps
         qctx.close()
+        big = S.config5()
         wctx = runtime.Context(big.params)
         wtime = {}
         for warm in (False, True):
@@ -537,7 +557,10 @@ def main():
                                            "euler_ms_per_step" % args.mode_steps,
             "warm_start_ms_per_step": round(wtime[True], 4),
             "cold_ms_per_step": round(wtime[False], 4),
-            "warm_start_workload": "config5 plain step (grid), first %d steps, wall time" % args.mode_steps}
+            "warm_start_workload": "config5 plain step (grid), first %d steps, wall time" % args.mode_steps,
+            "fp64_islands_ms_per_step": round(modes["islands"], 4),
+            "fp64_islands_workload": "config3 batched step with GJK termination and EPA's final face and depth "
+                                     "in binary64, %d steps, kernel time against euler_ms_per_step" % args.mode_steps}
     if not args.no_cpu:
         procs = args.cpu_procs or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds, procs, 240)
