@@ -198,7 +198,11 @@ int nans_upload_worlds(nans_ctx *ctx, int32_t nworlds, const int32_t *world_offs
 int nans_download_state(nans_ctx *ctx, nans_bodies_soa *bodies);
 
 /* Advance every world by nsteps with the plain path: one world per launch
- * sequence (the drop-in for the reference's physics block). */
+ * sequence (the drop-in for the reference's physics block).
+ * Both stepping calls (and nans_rollout) return NANS_ERR_CAPACITY when a step
+ * had more pair candidates or contacts than its scratch holds; the batch is
+ * then left as it was before the call (state, force, torque, quaternions and
+ * statistics), with the warm-start lists emptied. */
 int nans_step_plain(nans_ctx *ctx, int32_t nsteps);
 
 /* Advance every world by nsteps with the batched path: one persistent launch per

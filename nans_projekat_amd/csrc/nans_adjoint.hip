@@ -354,6 +354,7 @@ int nans_rollout(nans_ctx *ctx, int32_t nsteps, int32_t checkpoint_every)
     }
     if (!T->last) NANS_TRY(dmalloc(&T->last, slot));
     T->steps = 0;
+    NANS_TRY(snapshot_take(ctx, s));
     const int block = batched_block(ctx);
     for (int j = 0; j < ncp; ++j) {
         const int len = std::min(checkpoint_every, nsteps - j * checkpoint_every);
@@ -362,7 +363,7 @@ int nans_rollout(nans_ctx *ctx, int32_t nsteps, int32_t checkpoint_every)
     }
     NANS_TRY(save_state(W, T->last, s));
     NANS_HIP(hipStreamSynchronize(s));
-    NANS_TRY(check_overflow(ctx));
+    NANS_TRY(step_finish(ctx));
     T->steps = nsteps;
     T->every = checkpoint_every;
     T->params = ctx->p;

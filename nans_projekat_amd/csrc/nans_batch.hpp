@@ -89,6 +89,11 @@ StepConsts step_consts(const nans_params &p, int orientation);
 int warm_reset(nans_ctx *ctx);
 int setup_floor(nans_ctx *ctx, hipStream_t s);
 int check_overflow(nans_ctx *ctx);
+// Saves the state a stepping call starts from; step_finish checks the call for a
+// capacity overflow and, if it overflowed, restores that state and the counters
+// (and empties the warm-start lists) before returning NANS_ERR_CAPACITY.
+int snapshot_take(nans_ctx *ctx, hipStream_t s);
+int step_finish(nans_ctx *ctx);
 // nsteps batched steps replayed from the captured one-step graph
 int batched_run(nans_ctx *ctx, int nsteps, int block);
 // threads per world of the batched step (the autotuner's choice or the default)
@@ -186,4 +191,5 @@ struct nans_worlds {
     float timed_ms = 0.0f;
     int tuned_block = 0;   // autotuner's choice of threads per world (0: not tuned)
     nans::AdjointTape *adj = nullptr;  // checkpoints and scratch of the differentiable rollout
+    float *snap = nullptr;             // state at the start of a stepping call (snapshot_take)
 };

@@ -88,16 +88,18 @@ struct PairSrc {
 // runs 5.1 ms against 6.8 ms at two waves per SIMD and 1024-pair ranges.
 constexpr int NP_RANGE = 256;
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_narrowphase(int64_t n, NarrowIn a, NarrowIn b, NarrowParams p,
-                                                     NarrowOutDev o, unsigned long long *overflow, int *qidx,
-                                                     SimplexRec *qsim)
+// ISL: the FP64-islands mode (grouped EPA tiers with a binary64 final face).
+template <bool ISL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_narrowphase(
+    int64_t n, NarrowIn a, NarrowIn b, NarrowParams p, NarrowOutDev o, unsigned long long *overflow, int *qidx,
+    SimplexRec *qsim)
 {
     __shared__ EpaLds lds[4];
     __shared__ NarrowShared sh;
     const int64_t base = (int64_t)blockIdx.x * NP_RANGE;
     const int count = (int)min((int64_t)NP_RANGE, n - base);
     PairSrc src{a, b, base, n, o, overflow};
-    narrow_block<256, true>(src, count, p, lds, sh, qidx + base, qsim + base);
+    narrow_block<256, true, ISL>(src, count, p, lds, sh, qidx + base, qsim + base);
     if (threadIdx.x == 0 && sh.stalled) atomicAdd(overflow, (unsigned long long)sh.stalled);
 }
 
@@ -302,7 +304,10 @@ static int narrow_launch(nans_ctx *ctx, int64_t n, const nans_shapes_soa *a, con
         NarrowIn ib{b->type, b->pos, b->quat, b->half};
         NarrowOutDev o{out->gjk, out->contact, out->status, out->normal, out->point_a, out->point_b, out->depth};
         unsigned grid = (unsigned)((n + NP_RANGE - 1) / NP_RANGE);
-        k_narrowphase<<<grid, 256, 0, s>>>(n, ia, ib, narrow_params(ctx->p), o, cnt, qidx, qsim);
+        if (ctx->p.fp64_islands)
+            k_narrowphase<true><<<grid, 256, 0, s>>>(n, ia, ib, narrow_params(ctx->p), o, cnt, qidx, qsim);
+        else
+            k_narrowphase<false><<<grid, 256, 0, s>>>(n, ia, ib, narrow_params(ctx->p), o, cnt, qidx, qsim);
         NANS_HIP(hipGetLastError());
     }
     unsigned long long ov = 0;

@@ -728,27 +728,39 @@ __device__ inline EpaShapes epa_shapes_group(const Shape &A, const Shape &B)
     return e;
 }
 
-// CalculateSupport for each group's pair along its d (group-uniform): the corner
-// argmax runs in every row; a G-lane group reads the result of its first row.
+// CalculateSupport for each group's pair along its d (group-uniform).  Every row
+// of 16 lanes holds the corners of A (lanes 0-7) and B (8-15); a butterfly of DPP
+// moves inside each 8-lane half carries (dot, corner number, corner) to the first
+// corner of greatest dot product -- GetCubeSupport's strict '>' scan in corner
+// order -- and a row mirror hands each half the other half's winner.  No
+// cross-lane traffic leaves the row, so a 32-lane group's two rows agree.
 template <int G>
 __device__ inline Vtx group_support(const EpaShapes &s, V3 d)
 {
-    const int lane = lane_id(), j = lane & 15, gb = lane & (64 - G);
+    const int lane = lane_id(), j = lane & 15;
     const V3 nd = mul(d, -1.0f);
     const float dist = dot(s.k, j >= 8 ? nd : d);
     float key = dist > -FLT_MAX ? dist : -__builtin_inff();
     int idx = j & 7;
-    auto step = [&](float ok, int oi) {
+    V3 cw = s.k;
+    auto step = [&](float ok, int oi, float ox, float oy, float oz) {
         const bool take = ok > key || (ok == key && oi < idx);
         key = take ? ok : key;
         idx = take ? oi : idx;
+        cw.x = take ? ox : cw.x;
+        cw.y = take ? oy : cw.y;
+        cw.z = take ? oz : cw.z;
     };
-    step(dpp_f<0xB1>(key), dpp_i<0xB1>(idx));
-    step(dpp_f<0x4E>(key), dpp_i<0x4E>(idx));
-    step(dpp_f<0x141>(key), dpp_i<0x141>(idx));
-    const float ka = gpull_f(key, gb), kb = gpull_f(key, gb + 8);
-    const int wa = gb + gpull_i(idx, gb), wb = gb + 8 + gpull_i(idx, gb + 8);
-    const V3 ca = gpull3(s.k, wa), cb = gpull3(s.k, wb);
+    step(dpp_f<0xB1>(key), dpp_i<0xB1>(idx), dpp_f<0xB1>(cw.x), dpp_f<0xB1>(cw.y), dpp_f<0xB1>(cw.z));
+    step(dpp_f<0x4E>(key), dpp_i<0x4E>(idx), dpp_f<0x4E>(cw.x), dpp_f<0x4E>(cw.y), dpp_f<0x4E>(cw.z));
+    step(dpp_f<0x141>(key), dpp_i<0x141>(idx), dpp_f<0x141>(cw.x), dpp_f<0x141>(cw.y), dpp_f<0x141>(cw.z));
+    // this half's winner and, through the row mirror, the other half's
+    const float ko = dpp_f<0x140>(key);
+    const V3 co = mk3(dpp_f<0x140>(cw.x), dpp_f<0x140>(cw.y), dpp_f<0x140>(cw.z));
+    const bool onb = j >= 8;
+    const float ka = onb ? ko : key, kb = onb ? key : ko;
+    const V3 ca = mk3(onb ? co.x : cw.x, onb ? co.y : cw.y, onb ? co.z : cw.z);
+    const V3 cb = mk3(onb ? cw.x : co.x, onb ? cw.y : co.y, onb ? cw.z : co.z);
     Vtx v;
     v.A = s.kind_a == SHAPE_BOX ? (ka == -__builtin_inff() ? mk3(0.0f, 0.0f, 0.0f) : ca)
                                 : sphere_support(s.ca, s.ra, d);
@@ -767,7 +779,40 @@ __device__ __forceinline__ int edge_key(int cl, int e)
 // Every lane calls; lanes 0-3 of a group pass the supports of simplex vertex j in
 // sa0 / sb0.  act: the group has a pair.  Returns the group's result (EPA_NONE for
 // an idle group, EPA_SPILL to redo with the next wider tier).
+// Binary64 distance of a group face's plane to the origin (FP64-islands final
+// choice), its vertices b and c from the group's vertex LDS.
+__device__ inline double group_face_dist64(const float (*vp)[3], const FaceR &f)
+{
+    const int b = (f.id >> 8) & 255, c = (f.id >> 16) & 255;
+    const double ax = f.ax, ay = f.ay, az = f.az;
+    const double ux = (double)vp[b][0] - ax, uy = (double)vp[b][1] - ay, uz = (double)vp[b][2] - az;
+    const double vx = (double)vp[c][0] - ax, vy = (double)vp[c][1] - ay, vz = (double)vp[c][2] - az;
+    const double nx = uy * vz - vy * uz, ny = uz * vx - vz * ux, nz = ux * vy - vx * uy;
+    const double inv = 1.0 / __builtin_sqrt((nx * nx + ny * ny) + nz * nz);
+    const double d = ((nx * inv) * ax + (ny * inv) * ay) + (nz * inv) * az;
+    return d >= 0.0 ? d : -d;
+}
+
+// Argmin over (value, index) in each G-lane group, ties to the lowest index;
+// every lane of the group receives the result.
 template <int G>
+__device__ inline void group_argmin64(bool &valid, double &v, int &i)
+{
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+        const double ov = __shfl_xor(v, off, 64);
+        const int oi = __shfl_xor(i, off, 64);
+        const int ok = __shfl_xor(valid ? 1 : 0, off, 64);
+        const bool take = ok && (!valid || ov < v || (ov == v && oi < i));
+        v = take ? ov : v;
+        i = take ? oi : i;
+        valid = valid || ok;
+    }
+}
+
+// ISL: the FP64-islands mode, whose final face choice and depth are binary64
+// (as epa_wave's).
+template <int G, bool ISL = false>
 __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0, V3 sb0, bool act,
                                       const NarrowParams &p)
 {
@@ -831,19 +876,43 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
         const V3 N = mk3(gpull_f(F.nx, owner), gpull_f(F.ny, owner), gpull_f(F.nz, owner));
         const float dsel = gpull_f(F.d, owner);
         const float cur = m == __builtin_inff() ? d0 : dsel;
-        const int fid = gpull_i(F.id, owner);
+        int fid = gpull_i(F.id, owner);
         // new support along the stored normal
         const Vtx ns = group_support<G>(sh, N);
         const bool conv = run && dot(N, ns.P) - cur < p.epa_tolerance;
         if (ballot(conv)) {
-            // Barycentric (code/nans.cpp:771-785) of N * d on the chosen face
+            V3 Nc = N;
+            float cc = cur;
+            double depth = (double)cur;
+            if constexpr (ISL) {
+                // the final face by binary64 plane distance: the first of the least
+                // below face 0's, else face 0 (epa_wave's islands choice per group)
+                wave_sync();
+                const double d64 = j < nf ? group_face_dist64(L.vp[g], F) : 0.0;
+                const double f0 = __shfl(d64, gb, 64);
+                bool v64 = conv && j < nf && d64 < f0;
+                double bd = v64 ? d64 : 0.0;
+                int bi = v64 ? j : 0x7fffffff;
+                group_argmin64<G>(v64, bd, bi);
+                const int sel = v64 ? bi : 0;
+                depth = v64 ? bd : f0;
+                const int so = gb + sel;
+                const V3 Ns = mk3(gpull_f(F.nx, so), gpull_f(F.ny, so), gpull_f(F.nz, so));
+                const float ds = gpull_f(F.d, so);
+                const int fs = gpull_i(F.id, so);
+                if (sel != best) {
+                    Nc = Ns;
+                    cc = sel == 0 ? d0 : ds;
+                    fid = fs;
+                }
+            }
             const int a = fid & 255, b = (fid >> 8) & 255, c = (fid >> 16) & 255;
             const int am = gb + (a & (G - 1)), bm = gb + (b & (G - 1)), cm = gb + (c & (G - 1));
             const V3 PA = gpull3(VP, am), PB = gpull3(VP, bm), PC = gpull3(VP, cm);
             const V3 AA = gpull3(VA, am), AB = gpull3(VA, bm), AC = gpull3(VA, cm);
             const V3 BA = gpull3(VB, am), BB = gpull3(VB, bm), BC = gpull3(VB, cm);
             if (conv) {
-                const V3 P = mul(N, cur);
+                const V3 P = mul(Nc, cc);
                 const V3 e0 = sub(PB, PA), e1 = sub(PC, PA), e2 = sub(P, PA);
                 const float d00 = dot(e0, e0), d01 = dot(e0, e1), d11 = dot(e1, e1);
                 const float d20 = dot(e2, e0), d21 = dot(e2, e1);
@@ -857,9 +926,9 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
                     r.status = EPA_NONFINITE;
                 } else {
                     r.pa = add(add(mul(AA, U), mul(AB, V)), mul(AC, W));
-                    r.n = mul(N, -1.0f);
+                    r.n = mul(Nc, -1.0f);
                     r.pb = add(add(mul(BA, U), mul(BB, V)), mul(BC, W));
-                    r.depth = (double)cur;
+                    r.depth = depth;
                     r.status = EPA_CONTACT;
                 }
                 run = false;
@@ -1056,6 +1125,85 @@ __device__ __forceinline__ void simplex_rec_put(SimplexRec &q, const Vtx &v, int
     q.b[3 * k] = v.B.x; q.b[3 * k + 1] = v.B.y; q.b[3 * k + 2] = v.B.z;
 }
 
+__device__ __forceinline__ void simplex_rec_store(SimplexRec &q, const Simplex &S)
+{
+    simplex_rec_put(q, S.s0, 0);
+    simplex_rec_put(q, S.s1, 1);
+    simplex_rec_put(q, S.s2, 2);
+    simplex_rec_put(q, S.s3, 3);
+}
+
+// One lane's GJK loop (CheckCollision's EvolveSimplex calls, code/nans.cpp:906-966)
+// over the candidates it is handed.
+//
+// GJK cycles: EvolveSimplex's next call depends only on the simplex's Minkowski
+// points (SupA/SupB ride along), so once the points repeat bit for bit the calls
+// repeat for ever and the loop can only end at the cap.  Each lane keeps one
+// earlier state (Brent's scheme, started at call GJK_CYCLE_START: saved after that
+// call and then 1, 2, 4, ... calls later) in cyc[], its wave's [16][64] floats
+// (P of s0..s3, n, then the pair's first direction).  Most loops end before the
+// start, and pay nothing for the check.
+struct GjkLane {
+    Shape A, B;
+    Simplex S;
+    int c = -1, calls = 0, power = 1, lam = 0;
+    bool saved = false, retired = false;
+};
+
+// Candidate c for the lane (c >= ncand: none left, the lane retires).
+template <class Src>
+__device__ inline void gjk_lane_take(GjkLane &G, const Src &src, int c, int ncand, float *cyc)
+{
+    if (c >= ncand) {
+        G.retired = true;
+        return;
+    }
+    const int tid = lane_id();
+    G.c = c;
+    src.shapes(c, G.A, G.B);
+    const V3 d0 = gjk_first_dir(G.A, G.B);  // kept beside the cycle state
+    cyc[13 * 64 + tid] = d0.x;
+    cyc[14 * 64 + tid] = d0.y;
+    cyc[15 * 64 + tid] = d0.z;
+    G.S.n = 0;
+    G.calls = 0;
+    G.saved = false;
+}
+
+// One EvolveSimplex call of the lane's candidate, with the cycle check: the
+// outcome once the loop has ended (GJK_NO, GJK_FOUND, or GJK_CAP at the 65-call
+// cap), else GJK_STILL.
+__device__ inline int gjk_lane_call(GjkLane &G, float *cyc, const NarrowParams &np)
+{
+    const int tid = lane_id();
+    Simplex &S = G.S;
+    const V3 d0 = mk3(cyc[13 * 64 + tid], cyc[14 * 64 + tid], cyc[15 * 64 + tid]);
+    int res = evolve_simplex(G.A, G.B, S, np.fp64_islands, d0);
+    ++G.calls;
+    if (res == GJK_STILL && G.calls <= np.max_gjk_iterations && G.calls >= GJK_CYCLE_START) {
+        const float st[13] = {S.s0.P.x, S.s0.P.y, S.s0.P.z, S.s1.P.x, S.s1.P.y, S.s1.P.z,
+                              S.s2.P.x, S.s2.P.y, S.s2.P.z, S.s3.P.x, S.s3.P.y, S.s3.P.z,
+                              __int_as_float(S.n)};
+        if (G.saved) {
+            bool same = __float_as_int(cyc[12 * 64 + tid]) == S.n;
+#pragma unroll
+            for (int k = 0; k < 12; ++k)
+                same = same && (k >= 3 * S.n || __float_as_int(cyc[k * 64 + tid]) == __float_as_int(st[k]));
+            if (same) res = GJK_CAP;  // a repeated state: the cap is certain
+        }
+        ++G.lam;
+        if (res == GJK_STILL && (!G.saved || G.lam == G.power)) {
+#pragma unroll
+            for (int k = 0; k < 13; ++k) cyc[k * 64 + tid] = st[k];
+            G.power = G.saved ? 2 * G.power : 1;
+            G.saved = true;
+            G.lam = 0;
+        }
+    }
+    if (res != GJK_STILL || G.calls > np.max_gjk_iterations) return res == GJK_STILL ? GJK_CAP : res;
+    return GJK_STILL;
+}
+
 // Source interface (duck-typed):
 //   static constexpr bool kMemCorners                  box corners readable in memory:
 //   void shapes_ref(int c, Shape &A, Shape &B) const   then both shapes of candidate c
@@ -1065,10 +1213,13 @@ __device__ __forceinline__ void simplex_rec_put(SimplexRec &q, const Vtx &v, int
 //   void epa_result(int c, const EpaResult &r) const    every GJK hit (lane 0 calls)
 // WIDE: polytopes that outgrow a 16-lane group go to a 32-lane group before the
 // whole wave (worth its registers where many do, as among random deep overlaps).
-template <int BLOCK, bool WIDE, class Src>
+// ISL: the FP64-islands mode runs the grouped tiers too (their final face choice
+// in binary64); without it, that mode runs every polytope by the whole wave.
+template <int BLOCK, bool WIDE, bool ISL = false, class Src>
 __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, EpaLds *epa, NarrowShared &sh,
                              int *qidx, SimplexRec *qsim)
 {
+    const bool grouped = ISL || !np.fp64_islands;
     const int lane = lane_id();
     if (threadIdx.x == 0) {
         sh.next = 0;
@@ -1077,89 +1228,35 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
         sh.gjk_done = 0;
         sh.stalled = 0;
     }
-    if (!np.fp64_islands)
+    if (grouped)
         for (int k = threadIdx.x; k < ncand; k += BLOCK) qidx[k] = -1;  // queue slots: unpublished
     __syncthreads();
     {
-        // GJK cycles: EvolveSimplex's next call depends only on the simplex's
-        // Minkowski points (SupA/SupB ride along), so once the points repeat bit for
-        // bit the calls repeat for ever and the loop can only end at the cap.  Each
-        // lane keeps one earlier state (Brent's scheme, started at call
-        // GJK_CYCLE_START: saved after that call and then 1, 2, 4, ... calls
-        // later) in its wave's EPA LDS, which is idle until the wave's GJK ends.
-        // Most loops end before the start, and pay nothing for the check.
-        // the wave's own: [16][64], P of s0..s3, n, then the pair's first direction
+        // GJK lanes (gjk_lane_call); the cycle state lives in the wave's EPA LDS,
+        // idle until the wave's GJK ends
         float *cyc = reinterpret_cast<float *>(&epa[threadIdx.x >> 6]);
-        const int tid = lane;
-        Shape A, B;
-        Simplex S;
-        S.n = 0;
-        int c = -1, calls = 0;
-        int power = 1, lam = 0;
-        bool saved = false;
-        bool retired = false;
+        GjkLane G;
         for (;;) {
-            const bool need = c < 0 && !retired;
+            const bool need = G.c < 0 && !G.retired;
             const uint64_t m = ballot(need);
-            if (m && (__popcll(m) >= GJK_REFILL_MIN || !ballot(c >= 0))) {
+            if (m && (__popcll(m) >= GJK_REFILL_MIN || !ballot(G.c >= 0))) {
                 int base = 0;
                 if (lane == 0) base = atomicAdd(&sh.next, __popcll(m));
                 base = __builtin_amdgcn_readfirstlane(base);
-                if (need) {
-                    const int my = base + __popcll(m & ((1ull << lane) - 1ull));
-                    if (my < ncand) {
-                        c = my;
-                        src.shapes(c, A, B);
-                        const V3 d0 = gjk_first_dir(A, B);  // kept beside the cycle state
-                        cyc[13 * 64 + tid] = d0.x;
-                        cyc[14 * 64 + tid] = d0.y;
-                        cyc[15 * 64 + tid] = d0.z;
-                        S.n = 0;
-                        calls = 0;
-                        saved = false;
-                    } else {
-                        retired = true;
-                    }
-                }
+                if (need) gjk_lane_take(G, src, base + __popcll(m & ((1ull << lane) - 1ull)), ncand, cyc);
             }
-            if (!ballot(c >= 0)) break;
-            if (c >= 0) {
-                const V3 d0 = mk3(cyc[13 * 64 + tid], cyc[14 * 64 + tid], cyc[15 * 64 + tid]);
-                int res = evolve_simplex(A, B, S, np.fp64_islands, d0);
-                ++calls;
-                if (res == GJK_STILL && calls <= np.max_gjk_iterations && calls >= GJK_CYCLE_START) {
-                    const float st[13] = {S.s0.P.x, S.s0.P.y, S.s0.P.z, S.s1.P.x, S.s1.P.y, S.s1.P.z,
-                                          S.s2.P.x, S.s2.P.y, S.s2.P.z, S.s3.P.x, S.s3.P.y, S.s3.P.z,
-                                          __int_as_float(S.n)};
-                    if (saved) {
-                        bool same = __float_as_int(cyc[12 * 64 + tid]) == S.n;
-#pragma unroll
-                        for (int k = 0; k < 12; ++k)
-                            same = same && (k >= 3 * S.n || __float_as_int(cyc[k * 64 + tid]) == __float_as_int(st[k]));
-                        if (same) res = GJK_CAP;  // a repeated state: the cap is certain
-                    }
-                    ++lam;
-                    if (res == GJK_STILL && (!saved || lam == power)) {
-#pragma unroll
-                        for (int k = 0; k < 13; ++k) cyc[k * 64 + tid] = st[k];
-                        power = saved ? 2 * power : 1;
-                        saved = true;
-                        lam = 0;
-                    }
-                }
-                if (res != GJK_STILL || calls > np.max_gjk_iterations) {
-                    const int g = res == GJK_STILL ? GJK_CAP : res;
-                    src.gjk_result(c, g);
+            if (!ballot(G.c >= 0)) break;
+            if (G.c >= 0) {
+                const int g = gjk_lane_call(G, cyc, np);
+                if (g != GJK_STILL) {
+                    src.gjk_result(G.c, g);
                     if (g == GJK_FOUND) {
                         const int q = atomicAdd(&sh.nepa, 1);
-                        simplex_rec_put(qsim[q], S.s0, 0);
-                        simplex_rec_put(qsim[q], S.s1, 1);
-                        simplex_rec_put(qsim[q], S.s2, 2);
-                        simplex_rec_put(qsim[q], S.s3, 3);
+                        simplex_rec_store(qsim[q], G.S);
                         // publish the slot after its simplex
-                        __hip_atomic_store(&qidx[q], c, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_store(&qidx[q], G.c, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
-                    c = -1;
+                    G.c = -1;
                 }
             }
         }
@@ -1167,7 +1264,7 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
         if (lane == 0) __hip_atomic_fetch_add(&sh.gjk_done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     EpaLds &L = epa[threadIdx.x >> 6];
-    if (!np.fp64_islands) {
+    if (grouped) {
         // four pairs per wave, one per 16-lane group, started while other waves
         // may still be in GJK; a spilled polytope is redone by the whole wave, or
         // with WIDE by a 32-lane group (two at a time) and, if it spills there
@@ -1227,7 +1324,7 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
             // WIDE: a pair with a sphere goes to the 32-lane tier at once (a sphere's
             // support gives a new vertex at every iteration, so its polytopes grow)
             const bool sph = WIDE && (A.kind != SHAPE_BOX || B.kind != SHAPE_BOX);
-            const EpaResult r = epa_group<16>(G, es, sa0, sb0, act && !sph, np);
+            const EpaResult r = epa_group<16, ISL>(G, es, sa0, sb0, act && !sph, np);
             if (act && !sph && j == 0 && r.status != EPA_SPILL) src.epa_result(c, r);
             const uint64_t spill = ballot(act && j == 0 && (sph || r.status == EPA_SPILL));
             wave_sync();
@@ -1250,7 +1347,7 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
                     b2 = mk3(q.b[3 * j2], q.b[3 * j2 + 1], q.b[3 * j2 + 2]);
                 }
                 wave_sync();
-                const EpaResult r2 = epa_group<32>(G2, e2, a2, b2, act2, np);
+                const EpaResult r2 = epa_group<32, ISL>(G2, e2, a2, b2, act2, np);
                 if (act2 && j2 == 0 && r2.status != EPA_SPILL) src.epa_result(cs, r2);
                 uint64_t spill2 = ballot(act2 && j2 == 0 && r2.status == EPA_SPILL);
                 wave_sync();

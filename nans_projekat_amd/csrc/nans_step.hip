@@ -181,17 +181,31 @@ struct WorldSrc {
 // SOLVE_REG * BLOCK are read from con[] each round.
 constexpr int SOLVE_REG = 2;
 
+// Contact c of a world's ordered list: a record of an array of them, or the
+// record of candidate idx[c] (the batched step keeps each contact where the
+// narrowphase wrote it and lists only the candidate numbers).
+struct DirectCon {
+    ContactRec *p;
+    __device__ ContactRec &operator()(int c) const { return p[c]; }
+};
+struct IndexedCon {
+    ContactRec *rec;
+    const int *idx;
+    __device__ ContactRec &operator()(int c) const { return rec[idx[c]]; }
+};
+
 // pre[0..npre) (may be empty) receives Constraint's position-only terms of the
 // first npre contacts in one parallel pass, so that the rounds only evaluate the
 // velocity-dependent part for them.
-template <int BLOCK, bool WARM = false>
-__device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int ncon, int *done, int *head,
+// con(c) is contact c's record (see DirectCon, IndexedCon).
+template <int BLOCK, bool WARM = false, class Con>
+__device__ inline void solve_rounds(const Planes &B, int n, const Con &con, int ncon, int *done, int *head,
                                     const FloorDev &fl, const StepConsts &k, float Cf, float Cr,
                                     ContactTerms *pre = nullptr, int npre = 0, float *warm = nullptr)
 {
     const int tid = threadIdx.x;
     npre = min(npre, ncon);
-    for (int c = tid; c < npre; c += BLOCK) contact_terms(B, con[c], fl, k, pre[c]);
+    for (int c = tid; c < npre; c += BLOCK) contact_terms(B, con(c), fl, k, pre[c]);
     // a thread's first SOLVE_REG contacts live in registers: their bodies, and
     // the impulses they apply, written back once after the rounds (no global
     // memory access on a round's critical path when their terms are in pre[])
@@ -202,8 +216,8 @@ __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int
     for (int r = 0; r < SOLVE_REG; ++r) {
         const int c = tid + r * BLOCK;
         pend[r] = c < ncon;
-        sa[r] = pend[r] ? con[c].slot_a : 0;
-        sb[r] = pend[r] ? con[c].slot_b : -1;
+        sa[r] = pend[r] ? con(c).slot_a : 0;
+        sb[r] = pend[r] ? con(c).slot_b : -1;
         dl[r][0] = dl[r][1] = dl[r][2] = 0.0f;
     }
     for (int c = tid + SOLVE_REG * BLOCK; c < ncon; c += BLOCK) done[c] = 0;
@@ -234,21 +248,21 @@ __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int
         }
         for (int c = tid + SOLVE_REG * BLOCK; c < ncon; c += BLOCK) {
             if (!done[c]) {
-                atomicMax(&hk[con[c].slot_a], key(c));
-                const int b = con[c].slot_b;
+                atomicMax(&hk[con(c).slot_a], key(c));
+                const int b = con(c).slot_b;
                 if (b >= 0) atomicMax(&hk[b], key(c));
             }
         }
         __syncthreads();
         int left = 0;
         auto apply = [&](int c) {
-            ContactRec r = con[c];
+            ContactRec r = con(c);
             float *sums = WARM ? warm + 3 * c : nullptr;
             if (c < npre) contact_apply(B, r, pre[c], k, Cf, Cr, sums);
             else constraint(B, r, fl, k, Cf, Cr, sums);
-            con[c].dln = r.dln;
-            con[c].dlt1 = r.dlt1;
-            con[c].dlt2 = r.dlt2;
+            con(c).dln = r.dln;
+            con(c).dlt1 = r.dlt1;
+            con(c).dlt2 = r.dlt2;
         };
 #pragma unroll
         for (int r = 0; r < SOLVE_REG; ++r) {
@@ -263,7 +277,7 @@ __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int
                         rr.slot_b = sb[r];
                         contact_apply(B, rr, pre[c], k, Cf, Cr, sums);
                     } else {
-                        rr = con[c];
+                        rr = con(c);
                         constraint(B, rr, fl, k, Cf, Cr, sums);
                     }
                     dl[r][0] = rr.dln;
@@ -277,7 +291,7 @@ __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int
         }
         for (int c = tid + SOLVE_REG * BLOCK; c < ncon; c += BLOCK) {
             if (!done[c]) {
-                const int a = con[c].slot_a, b = con[c].slot_b;
+                const int a = con(c).slot_a, b = con(c).slot_b;
                 if (hk[a] == key(c) && (b < 0 || hk[b] == key(c))) {
                     apply(c);
                     done[c] = 1;
@@ -292,9 +306,9 @@ __device__ inline void solve_rounds(const Planes &B, int n, ContactRec *con, int
     for (int r = 0; r < SOLVE_REG; ++r) {
         const int c = tid + r * BLOCK;
         if (c < ncon) {
-            con[c].dln = dl[r][0];
-            con[c].dlt1 = dl[r][1];
-            con[c].dlt2 = dl[r][2];
+            con(c).dln = dl[r][0];
+            con(c).dlt1 = dl[r][1];
+            con(c).dlt2 = dl[r][2];
         }
     }
 }
@@ -513,8 +527,8 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
         }
         __syncthreads();
     }
-    solve_rounds<SOLVE_THREADS, WARM>(a.B, a.n, a.con, ncon, a.done, head, fl, a.k, a.Cf, a.Cr, nullptr, 0,
-                                      a.warm_sums);
+    solve_rounds<SOLVE_THREADS, WARM>(a.B, a.n, DirectCon{a.con}, ncon, a.done, head, fl, a.k, a.Cf, a.Cr,
+                                      nullptr, 0, a.warm_sums);
     if (WARM) {
         __syncthreads();
         const int m = min(ncon, a.warm_cap);
@@ -566,7 +580,8 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
     int *done = flag;
     int *qidx = a.qidx + wg;
     SimplexRec *qsim = a.qsim + wg;
-    ContactRec *con = (ContactRec *)qsim;
+    ContactRec *con = (ContactRec *)qsim;  // warm start: the ordered records
+    int *cidx = qidx;                      // cold: the ordered contacts' candidate numbers
 
     for (;;) {
         if (tid == 0) s_job = atomicAdd(a.queue, 1);
@@ -681,9 +696,11 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
             atomicAdd(&a.cnt[CNT_GJK_FOUND], (unsigned long long)nsh.nepa);
             if (nsh.stalled) atomicAdd(&a.cnt[CNT_EPA_OVERFLOW], (unsigned long long)nsh.stalled);
         }
-        // the contacts in candidate (= canonical) order; con reuses the queue's memory
-        // (warm start: cand[] is compacted alongside, in place, to the contacts'
-        // canonical pair indices; block_rank's barriers order the reads before the writes)
+        // the contacts in candidate (= canonical) order: their candidate numbers in the
+        // queue's index memory (the records stay where the narrowphase wrote them);
+        // warm start: the records themselves in the queue's simplex memory, and cand[]
+        // compacted alongside, in place, to the contacts' canonical pair indices
+        // (block_rank's barriers order the reads before the writes)
         int ncon = 0;
         for (int base = 0; base < ncand; base += BLOCK) {
             int c = base + tid;
@@ -692,8 +709,12 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
             int tot;
             int rk = block_rank<BLOCK>(hit, wave_tot, tot);
             if (hit) {
-                con[ncon + rk] = rec[c];
-                if (WARM) cand[ncon + rk] = key;
+                if constexpr (WARM) {
+                    con[ncon + rk] = rec[c];
+                    cand[ncon + rk] = key;
+                } else {
+                    cidx[ncon + rk] = c;
+                }
             }
             ncon += tot;
         }
@@ -722,9 +743,13 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
         }
         const float Cf = a.wfric ? a.wfric[w] : a.k.friction;
         const float Cr = a.wrest ? a.wrest[w] : a.k.restitution;
-        solve_rounds<BLOCK, WARM>(Lb, n, con, ncon, done, headL, fl, a.k, Cf, Cr,
-                                  reinterpret_cast<ContactTerms *>(epa),
-                                  (int)(NW * sizeof(EpaLds) / sizeof(ContactTerms)), WARM ? wsum : nullptr);
+        const int npre = (int)(NW * sizeof(EpaLds) / sizeof(ContactTerms));
+        if constexpr (WARM)
+            solve_rounds<BLOCK, WARM>(Lb, n, DirectCon{con}, ncon, done, headL, fl, a.k, Cf, Cr,
+                                      reinterpret_cast<ContactTerms *>(epa), npre, wsum);
+        else
+            solve_rounds<BLOCK, WARM>(Lb, n, IndexedCon{rec, cidx}, ncon, done, headL, fl, a.k, Cf, Cr,
+                                      reinterpret_cast<ContactTerms *>(epa), npre, nullptr);
         if constexpr (WARM) {  // this step's list: the first warm_cap[w] contacts (contact c's sums are thread c % BLOCK's)
             const int64_t off = a.warm_off[w];
             const int m = min(ncon, a.warm_cap[w]);
@@ -744,7 +769,7 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
         if (a.rec_out) {
             ContactRec *dst = a.rec_out + a.rec_off[w];
             int rc = a.rec_cap[w];
-            for (int c = tid; c < ncon && c < rc; c += BLOCK) dst[c] = con[c];
+            for (int c = tid; c < ncon && c < rc; c += BLOCK) dst[c] = WARM ? con[c] : rec[cidx[c]];
             if (tid == 0) a.rec_count[w] = ncon;
         }
         __syncthreads();
@@ -825,7 +850,7 @@ int worlds_release(nans_ctx *ctx)
                     (void *)W->bp_bcnt, (void *)W->bp_boff, (void *)W->bp_counts, (void *)W->bp_offsets,
                     (void *)W->bp_tmp, (void *)W->bp_stash, (void *)W->bp_ovf, W->bp_grid, (void *)W->warm_p[0], (void *)W->warm_p[1], (void *)W->warm_s[0],
                     (void *)W->warm_s[1], (void *)W->warm_n[0], (void *)W->warm_n[1], (void *)W->warm_cp,
-                    (void *)W->warm_sums, (void *)W->warm_offd, (void *)W->warm_capd})
+                    (void *)W->warm_sums, (void *)W->warm_offd, (void *)W->warm_capd, (void *)W->snap})
         wfree(p);
     for (hipEvent_t ev : {W->ev_in, W->ev_out, W->ev_t0, W->ev_t1})
         if (ev) hipEventDestroy(ev);
@@ -916,8 +941,47 @@ int check_overflow(nans_ctx *ctx)
     unsigned long long c[CNT_N];
     NANS_HIP(hipMemcpy(c, ctx->worlds->cnt, sizeof(c), hipMemcpyDeviceToHost));
     NANS_REQUIRE(c[CNT_CAND_OVERFLOW] == 0 && c[CNT_CON_OVERFLOW] == 0, NANS_ERR_CAPACITY,
-                 "pair candidates or contacts exceeded the scratch capacity");
+                 "pair candidates or contacts exceeded the scratch capacity; the batch is left as it was "
+                 "before the call");
     return NANS_OK;
+}
+
+// A stepping call that drops candidates or contacts beyond its scratch has
+// advanced the state without them.  The call saves the state it starts from --
+// the planes [0, F_SX) (pos, vel, angvel, angles) and [F_FX, F_NFIELDS) (force,
+// torque, quaternions) and the counters -- and restores it when check_overflow
+// reports the overflow, so the error leaves the batch as the call found it.
+constexpr int SNAP_HEAD = F_SX, SNAP_TAIL = F_NFIELDS - F_FX;
+
+int snapshot_take(nans_ctx *ctx, hipStream_t s)
+{
+    nans_worlds *W = ctx->worlds;
+    const size_t N = (size_t)W->N;
+    if (!W->snap) NANS_TRY(dmalloc(&W->snap, (SNAP_HEAD + SNAP_TAIL) * N + 2 * CNT_N));
+    NANS_HIP(hipMemcpyAsync(W->snap, W->planes, sizeof(float) * SNAP_HEAD * N, hipMemcpyDeviceToDevice, s));
+    NANS_HIP(hipMemcpyAsync(W->snap + SNAP_HEAD * N, W->planes + (size_t)F_FX * N, sizeof(float) * SNAP_TAIL * N,
+                            hipMemcpyDeviceToDevice, s));
+    NANS_HIP(hipMemcpyAsync(W->snap + (SNAP_HEAD + SNAP_TAIL) * N, W->cnt, sizeof(unsigned long long) * CNT_N,
+                            hipMemcpyDeviceToDevice, s));
+    return NANS_OK;
+}
+
+// check_overflow, restoring the call's starting state when it fails.
+int step_finish(nans_ctx *ctx)
+{
+    const int rc = check_overflow(ctx);
+    if (rc != NANS_ERR_CAPACITY) return rc;
+    nans_worlds *W = ctx->worlds;
+    const size_t N = (size_t)W->N;
+    hipStream_t s = ctx->stream;
+    NANS_HIP(hipMemcpyAsync(W->planes, W->snap, sizeof(float) * SNAP_HEAD * N, hipMemcpyDeviceToDevice, s));
+    NANS_HIP(hipMemcpyAsync(W->planes + (size_t)F_FX * N, W->snap + SNAP_HEAD * N, sizeof(float) * SNAP_TAIL * N,
+                            hipMemcpyDeviceToDevice, s));
+    NANS_HIP(hipMemcpyAsync(W->cnt, W->snap + (SNAP_HEAD + SNAP_TAIL) * N, sizeof(unsigned long long) * CNT_N,
+                            hipMemcpyDeviceToDevice, s));
+    NANS_TRY(warm_reset(ctx));
+    NANS_HIP(hipStreamSynchronize(s));
+    return rc;
 }
 
 }  // namespace nans
@@ -1265,6 +1329,7 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
     hipStream_t s = ctx->stream;
     if (ctx->warm) NANS_TRY(warm_prepare(ctx));
     NANS_TRY(setup_floor(ctx, s));
+    NANS_TRY(snapshot_take(ctx, s));
     const StepConsts k = step_consts(ctx->p, ctx->orientation);
     for (int step = 0; step < nsteps; ++step) {
         for (int w = 0; w < W->nworlds; ++w) {
@@ -1331,7 +1396,7 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
         if (ctx->warm) W->warm_flip ^= 1;
     }
     NANS_HIP(hipStreamSynchronize(s));
-    return check_overflow(ctx);
+    return step_finish(ctx);
 }
 
 }  // extern "C"
@@ -1591,13 +1656,14 @@ int nans_step_batched(nans_ctx *ctx, int32_t nsteps)
     NANS_HIP(hipSetDevice(ctx->p.device));
     nans_worlds *W = ctx->worlds;
     int block = batched_block(ctx);
+    NANS_TRY(snapshot_take(ctx, ctx->stream));
     if (ctx->warm) NANS_TRY(batched_run_warm(ctx, nsteps, block));
     else NANS_TRY(batched_run(ctx, nsteps, block));
     NANS_HIP(hipStreamSynchronize(ctx->stream));
     float ms = 0.0f;
     NANS_HIP(hipEventElapsedTime(&ms, W->ev_t0, W->ev_t1));
     W->timed_ms = ms;
-    return check_overflow(ctx);
+    return step_finish(ctx);
 }
 
 int nans_batched_timing(nans_ctx *ctx, nans_timing *t)

@@ -102,3 +102,28 @@ def test_graph_dropped_when_scratch_grows():
         np.testing.assert_array_equal(ga[f], gb[f], err_msg=f)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("path", ["plain", "batched"])
+def test_capacity_overflow_leaves_the_batch_unchanged(path):
+    """A world of 512 coincident boxes has 130,816 candidate pairs, beyond both
+    steps' candidate scratch (plain: 64 n + 4096, batched: 32 n + 1024): the call
+    fails with NANS_ERR_CAPACITY and leaves the state and the statistics as they
+    were; a normal world beside it is untouched as well."""
+    n = 512
+    big = S._world(np.tile(np.float32([[3.0, 5.0, 2.0]]), (n, 1)), np.ones((n, 3), np.float32),
+                   np.ones(n, np.float32), np.zeros((0, 3)), np.zeros((0, 3)), [], [])
+    sc = S.concat([S.config1(), big])
+    sc.params = S.config1().params
+    ctx = runtime.Context(sc.params)
+    ctx.upload(sc)
+    before = ctx.download()
+    stats = ctx.stats()
+    with pytest.raises(runtime.NansError) as e:
+        (ctx.step_plain if path == "plain" else ctx.step_batched)(2)
+    assert e.value.code == runtime.ERR_CAPACITY
+    after = ctx.download()
+    for f in STATE:
+        np.testing.assert_array_equal(after[f], before[f], err_msg=f)
+    assert ctx.stats() == stats
+    ctx.close()
