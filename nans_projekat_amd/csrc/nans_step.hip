@@ -181,23 +181,16 @@ struct WorldSrc {
 // SOLVE_REG * BLOCK are read from con[] each round.
 constexpr int SOLVE_REG = 2;
 
-// Contact c of a world's ordered list: a record of an array of them, or the
-// record of candidate idx[c] (the batched step keeps each contact where the
-// narrowphase wrote it and lists only the candidate numbers).
+// Contact c of a world's ordered list.
 struct DirectCon {
     ContactRec *p;
     __device__ ContactRec &operator()(int c) const { return p[c]; }
-};
-struct IndexedCon {
-    ContactRec *rec;
-    const int *idx;
-    __device__ ContactRec &operator()(int c) const { return rec[idx[c]]; }
 };
 
 // pre[0..npre) (may be empty) receives Constraint's position-only terms of the
 // first npre contacts in one parallel pass, so that the rounds only evaluate the
 // velocity-dependent part for them.
-// con(c) is contact c's record (see DirectCon, IndexedCon).
+// con(c) is contact c's record (DirectCon).
 template <int BLOCK, bool WARM = false, class Con>
 __device__ inline void solve_rounds(const Planes &B, int n, const Con &con, int ncon, int *done, int *head,
                                     const FloorDev &fl, const StepConsts &k, float Cf, float Cr,
@@ -580,8 +573,7 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
     int *done = flag;
     int *qidx = a.qidx + wg;
     SimplexRec *qsim = a.qsim + wg;
-    ContactRec *con = (ContactRec *)qsim;  // warm start: the ordered records
-    int *cidx = qidx;                      // cold: the ordered contacts' candidate numbers
+    ContactRec *con = (ContactRec *)qsim;
 
     for (;;) {
         if (tid == 0) s_job = atomicAdd(a.queue, 1);
@@ -696,11 +688,9 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
             atomicAdd(&a.cnt[CNT_GJK_FOUND], (unsigned long long)nsh.nepa);
             if (nsh.stalled) atomicAdd(&a.cnt[CNT_EPA_OVERFLOW], (unsigned long long)nsh.stalled);
         }
-        // the contacts in candidate (= canonical) order: their candidate numbers in the
-        // queue's index memory (the records stay where the narrowphase wrote them);
-        // warm start: the records themselves in the queue's simplex memory, and cand[]
-        // compacted alongside, in place, to the contacts' canonical pair indices
-        // (block_rank's barriers order the reads before the writes)
+        // the contacts in candidate (= canonical) order; con reuses the queue's memory
+        // (warm start: cand[] is compacted alongside, in place, to the contacts'
+        // canonical pair indices; block_rank's barriers order the reads before the writes)
         int ncon = 0;
         for (int base = 0; base < ncand; base += BLOCK) {
             int c = base + tid;
@@ -709,12 +699,8 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
             int tot;
             int rk = block_rank<BLOCK>(hit, wave_tot, tot);
             if (hit) {
-                if constexpr (WARM) {
-                    con[ncon + rk] = rec[c];
-                    cand[ncon + rk] = key;
-                } else {
-                    cidx[ncon + rk] = c;
-                }
+                con[ncon + rk] = rec[c];
+                if (WARM) cand[ncon + rk] = key;
             }
             ncon += tot;
         }
@@ -743,13 +729,9 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
         }
         const float Cf = a.wfric ? a.wfric[w] : a.k.friction;
         const float Cr = a.wrest ? a.wrest[w] : a.k.restitution;
-        const int npre = (int)(NW * sizeof(EpaLds) / sizeof(ContactTerms));
-        if constexpr (WARM)
-            solve_rounds<BLOCK, WARM>(Lb, n, DirectCon{con}, ncon, done, headL, fl, a.k, Cf, Cr,
-                                      reinterpret_cast<ContactTerms *>(epa), npre, wsum);
-        else
-            solve_rounds<BLOCK, WARM>(Lb, n, IndexedCon{rec, cidx}, ncon, done, headL, fl, a.k, Cf, Cr,
-                                      reinterpret_cast<ContactTerms *>(epa), npre, nullptr);
+        solve_rounds<BLOCK, WARM>(Lb, n, DirectCon{con}, ncon, done, headL, fl, a.k, Cf, Cr,
+                                  reinterpret_cast<ContactTerms *>(epa),
+                                  (int)(NW * sizeof(EpaLds) / sizeof(ContactTerms)), WARM ? wsum : nullptr);
         if constexpr (WARM) {  // this step's list: the first warm_cap[w] contacts (contact c's sums are thread c % BLOCK's)
             const int64_t off = a.warm_off[w];
             const int m = min(ncon, a.warm_cap[w]);
@@ -769,7 +751,7 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
         if (a.rec_out) {
             ContactRec *dst = a.rec_out + a.rec_off[w];
             int rc = a.rec_cap[w];
-            for (int c = tid; c < ncon && c < rc; c += BLOCK) dst[c] = WARM ? con[c] : rec[cidx[c]];
+            for (int c = tid; c < ncon && c < rc; c += BLOCK) dst[c] = con[c];
             if (tid == 0) a.rec_count[w] = ncon;
         }
         __syncthreads();
