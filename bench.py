@@ -482,8 +482,9 @@ def main():
                                   "dt 1/120, RK4, 10 impulse iterations, batched step",
                       "worlds": sc.nworlds, "bodies": sc.nbodies, "parallelism": "none (1 GPU)",
                       "threads_per_world": timing["threads_per_world"], "workgroups": timing["workgroups"],
-                      "lds_bytes": timing["lds_bytes"],
-                      "autotune_ms_per_step": tuning["ms_per_step"] if tuning else None},
+                      "lds_bytes": timing["lds_bytes"], "wave_world_bodies": timing["wave_world_bodies"],
+                      "wave_worlds": timing["wave_worlds"],
+                      "autotune_ms_per_step": tuning["ms_per_step_wave"] if tuning else None},
            "roofline": roof,
            "step_counts": {k: stats[k] for k in ("pair_tests", "candidates", "gjk_found", "contacts")},
            "tolerance": "bit-exact with the binary32 CPU restatement of the reference step (tests/)"}

@@ -221,6 +221,8 @@ typedef struct nans_timing {
     int32_t threads_per_world;
     int32_t workgroups;
     int32_t lds_bytes;
+    int32_t wave_world_bodies;  /* worlds of at most this many bodies ran one per wave */
+    int32_t wave_worlds;        /* and there were this many */
 } nans_timing;
 int nans_batched_timing(nans_ctx *ctx, nans_timing *timing);
 
@@ -277,23 +279,32 @@ int nans_set_warm_start(nans_ctx *ctx, int32_t enable);
 int nans_set_broadphase(nans_ctx *ctx, int32_t mode);
 
 /* Autotuner of the batched step (no counterpart in the reference, which steps one
- * scene on one thread).  Each workgroup of the batched step runs one world: its
- * size sets how many canonical pairs are culled and GJK-tested at once, how many
- * EPA polytopes are in flight (one per wave) and the width of the solver's
- * dependency rounds.  nans_autotune_batched times nsteps steps of the uploaded
- * batch with 64, 128, 256 and 512 threads per world (sizes whose LDS does not fit
- * are skipped, ms_per_step = -1), restores the state, force/torque, quaternions
- * and statistics,
- * and keeps the fastest size for later nans_step_batched / nans_rollout calls.
- * Results do not depend on the size.  nans_set_batched_block forces a size
- * (0: the default, 128 threads when no world holds more than 64 bodies, else 256). */
+ * scene on one thread).  The batched step runs the larger worlds one per
+ * workgroup and, after them, the worlds of at most `wave-world bound` bodies one
+ * per wave (each wave of a workgroup stepping its own world).  The workgroup size
+ * sets how many canonical pairs of a world are culled and GJK-tested at once, how
+ * many EPA polytopes are in flight (one per wave) and the width of the solver's
+ * dependency rounds; the bound trades that width for more worlds in flight.
+ * nans_autotune_batched times nsteps steps of the uploaded batch for each
+ * workgroup size (64, 128, 256, 512 threads) and bound (0, 16, 32, 64, 128
+ * bodies; at 64 threads only 0), skipping combinations whose LDS does not fit or
+ * that split the worlds as the previous bound did (ms = -1); it restores the
+ * state, force/torque, quaternions and statistics, and keeps the fastest
+ * combination for later nans_step_batched / nans_rollout calls.  Results do not
+ * depend on either.  nans_set_batched_block forces a size (0: the default, 128
+ * threads when no world holds more than 64 bodies, else 256);
+ * nans_set_wave_worlds forces a bound (-1: the default, 0 = every world by a
+ * whole workgroup; at most 512). */
 typedef struct nans_tuning {
-    int32_t threads_per_world;   /* chosen */
-    int32_t steps;               /* steps timed per candidate */
-    float ms_per_step[4];        /* 64, 128, 256, 512 threads per world */
+    int32_t threads_per_world;      /* chosen size */
+    int32_t steps;                  /* steps timed per candidate */
+    float ms_per_step[4];           /* 64, 128, 256, 512 threads per world, each at its best bound */
+    int32_t wave_world_bodies;      /* chosen bound */
+    float ms_per_step_wave[4][5];   /* [size][bound]: bounds 0, 16, 32, 64, 128 bodies */
 } nans_tuning;
 int nans_autotune_batched(nans_ctx *ctx, int32_t nsteps, nans_tuning *out);
 int nans_set_batched_block(nans_ctx *ctx, int32_t threads_per_world);
+int nans_set_wave_worlds(nans_ctx *ctx, int32_t max_bodies);
 
 /* Keep the contact list of each world's last step (off by default); at most
  * 16 n + 64 contacts per world of n bodies are kept. */

@@ -36,8 +36,11 @@ struct BatchArgs {
     const int *nboxes;      // [nw]
     const int *order;       // largest world first
     int nworlds;
-    int nmax;               // LDS stride (>= largest world)
-    int *queue;             // this step's work counter
+    int nbig;               // order[0, nbig) run by a whole workgroup, the rest one per wave
+    int nmax;               // LDS stride of a workgroup's world (>= largest world)
+    int wrows;              // LDS stride of a wave's world (>= largest of order[nbig, nworlds))
+    int lds_floats;         // floats of world LDS before the polytope LDS (a multiple of 4)
+    int *queue;             // this step's work counters: [0] workgroup worlds, [1] wave worlds
     StepConsts k;
     const FloorDev *fl;
     const float *wfric, *wrest;  // per-world coefficients (may be null)
@@ -46,7 +49,8 @@ struct BatchArgs {
     int *flag;
     int *qidx;
     SimplexRec *qsim;
-    int cap;
+    int cap;                // entries per workgroup
+    int wcap;               // entries per wave in the wave phase (cap / waves)
     ContactRec *rec_out;    // recording (may be null)
     const int64_t *rec_off;
     int *rec_count;
@@ -187,9 +191,12 @@ struct nans_worlds {
     int graph_orientation = 0;
     hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
     int graph_grid = 0, graph_lds = 0;
+    int graph_wave = -1;     // worlds of at most this many bodies ran one per wave
+    int graph_nwave = 0;     // and there were this many of them
     int timed_steps = 0;
     float timed_ms = 0.0f;
     int tuned_block = 0;   // autotuner's choice of threads per world (0: not tuned)
+    int tuned_wave = -1;   // autotuner's or caller's wave-world bound in bodies (-1: default)
     nans::AdjointTape *adj = nullptr;  // checkpoints and scratch of the differentiable rollout
     float *snap = nullptr;             // state at the start of a stepping call (snapshot_take)
 };

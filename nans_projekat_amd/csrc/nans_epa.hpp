@@ -67,6 +67,39 @@ __device__ __forceinline__ void wave_sync()
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// The threads that run one world's step or one narrowphase: the whole workgroup,
+// or one wave of it.  whole must be uniform over the workgroup (made from
+// constants or readfirstlane values), so that sync()'s barrier is reached by
+// every wave or by none.  A wave team's sync orders its global and LDS accesses
+// at workgroup scope (the memory it shares is the workgroup's), with no barrier.
+struct Team {
+    int rank;    // the thread's index in the team
+    int size;    // threads in the team: the workgroup's, or 64
+    bool whole;  // the whole workgroup
+
+    __device__ static Team block(int threads) { return Team{(int)threadIdx.x, threads, true}; }
+    __device__ static Team wave() { return Team{(int)(threadIdx.x & 63), 64, false}; }
+    __device__ int waves() const { return size >> 6; }
+    __device__ void sync() const
+    {
+        if (whole) {
+            __syncthreads();
+        } else {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        }
+    }
+    // sync() that also returns the number of threads with p set
+    __device__ int sync_count(bool p) const
+    {
+        if (whole) return __syncthreads_count(p);
+        const int c = __popcll(__ballot(p));
+        sync();
+        return c;
+    }
+};
 __device__ __forceinline__ int ffs64(uint64_t m) { return __builtin_ctzll(m); }
 
 __device__ __forceinline__ V3 ld3(const float (*a)[3], int i) { return mk3(a[i][0], a[i][1], a[i][2]); }
@@ -1215,13 +1248,14 @@ __device__ inline int gjk_lane_call(GjkLane &G, float *cyc, const NarrowParams &
 // whole wave (worth its registers where many do, as among random deep overlaps).
 // ISL: the FP64-islands mode runs the grouped tiers too (their final face choice
 // in binary64); without it, that mode runs every polytope by the whole wave.
-template <int BLOCK, bool WIDE, bool ISL = false, class Src>
-__device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, EpaLds *epa, NarrowShared &sh,
-                             int *qidx, SimplexRec *qsim)
+// The team's wave k uses epa[k]; sh, qidx and qsim are the team's own.
+template <bool WIDE, bool ISL = false, class Src>
+__device__ void narrow_team(const Team &team, const Src &src, int ncand, const NarrowParams &np, EpaLds *epa,
+                            NarrowShared &sh, int *qidx, SimplexRec *qsim)
 {
     const bool grouped = ISL || !np.fp64_islands;
     const int lane = lane_id();
-    if (threadIdx.x == 0) {
+    if (team.rank == 0) {
         sh.next = 0;
         sh.nepa = 0;
         sh.epa_next = 0;
@@ -1229,12 +1263,12 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
         sh.stalled = 0;
     }
     if (grouped)
-        for (int k = threadIdx.x; k < ncand; k += BLOCK) qidx[k] = -1;  // queue slots: unpublished
-    __syncthreads();
+        for (int k = team.rank; k < ncand; k += team.size) qidx[k] = -1;  // queue slots: unpublished
+    team.sync();
     {
         // GJK lanes (gjk_lane_call); the cycle state lives in the wave's EPA LDS,
         // idle until the wave's GJK ends
-        float *cyc = reinterpret_cast<float *>(&epa[threadIdx.x >> 6]);
+        float *cyc = reinterpret_cast<float *>(&epa[team.rank >> 6]);
         GjkLane G;
         for (;;) {
             const bool need = G.c < 0 && !G.retired;
@@ -1263,13 +1297,13 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
         // this wave's queue slots are published before it counts itself out
         if (lane == 0) __hip_atomic_fetch_add(&sh.gjk_done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-    EpaLds &L = epa[threadIdx.x >> 6];
+    EpaLds &L = epa[team.rank >> 6];
     if (grouped) {
         // four pairs per wave, one per 16-lane group, started while other waves
         // may still be in GJK; a spilled polytope is redone by the whole wave, or
         // with WIDE by a 32-lane group (two at a time) and, if it spills there
         // too, by the whole wave
-        constexpr int NW = BLOCK / 64;
+        const int NW = team.waves();
         EpgLds<16> &G = *reinterpret_cast<EpgLds<16> *>(&L);
         EpgLds<32> &G2 = *reinterpret_cast<EpgLds<32> *>(&L);
         const int j = lane & 15;
@@ -1397,10 +1431,10 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
                 wave_sync();
             }
         }
-        __syncthreads();
+        team.sync();
         return;
     }
-    __syncthreads();
+    team.sync();
     const int nepa = sh.nepa;
     for (;;) {
         int job = 0;
@@ -1426,7 +1460,15 @@ __device__ void narrow_block(const Src &src, int ncand, const NarrowParams &np, 
         if (lane == 0) src.epa_result(c, r);
         wave_sync();
     }
-    __syncthreads();
+    team.sync();
+}
+
+// narrow_team by the whole workgroup of BLOCK threads
+template <int BLOCK, bool WIDE, bool ISL = false, class Src>
+__device__ inline void narrow_block(const Src &src, int ncand, const NarrowParams &np, EpaLds *epa,
+                                    NarrowShared &sh, int *qidx, SimplexRec *qsim)
+{
+    narrow_team<WIDE, ISL>(Team::block(BLOCK), src, ncand, np, epa, sh, qidx, qsim);
 }
 
 }  // namespace nans

@@ -36,24 +36,34 @@ namespace nans {
 
 
 // ------------------------------------------------------------------ small block helpers
-template <int BLOCK>
-__device__ inline int block_rank(bool pred, int *wave_tot, int &total)
+// The thread's rank among the team's threads with pred set, and their number
+// (wave_tot: one int per wave of the team).
+__device__ inline int team_rank(const Team &t, bool pred, int *wave_tot, int &total)
 {
-    const int lane = lane_id(), w = threadIdx.x >> 6;
+    const int lane = lane_id(), w = t.rank >> 6;
     uint64_t m = ballot(pred);
     int r = __popcll(m & ((1ull << lane) - 1ull));
+    if (!t.whole) {
+        total = __popcll(m);
+        return r;
+    }
     if (lane == 0) wave_tot[w] = __popcll(m);
     __syncthreads();
     int off = 0, tot = 0;
-#pragma unroll
-    for (int i = 0; i < BLOCK / 64; ++i) {
-        int t = wave_tot[i];
-        off += i < w ? t : 0;
-        tot += t;
+    for (int i = 0; i < t.waves(); ++i) {
+        int c = wave_tot[i];
+        off += i < w ? c : 0;
+        tot += c;
     }
     __syncthreads();
     total = tot;
     return off + r;
+}
+
+template <int BLOCK>
+__device__ inline int block_rank(bool pred, int *wave_tot, int &total)
+{
+    return team_rank(Team::block(BLOCK), pred, wave_tot, total);
 }
 
 // A body's shape without its corner registers: kind, centre, radius and vm.
@@ -191,12 +201,12 @@ struct DirectCon {
 // first npre contacts in one parallel pass, so that the rounds only evaluate the
 // velocity-dependent part for them.
 // con(c) is contact c's record (DirectCon).
-template <int BLOCK, bool WARM = false, class Con>
-__device__ inline void solve_rounds(const Planes &B, int n, const Con &con, int ncon, int *done, int *head,
-                                    const FloorDev &fl, const StepConsts &k, float Cf, float Cr,
-                                    ContactTerms *pre = nullptr, int npre = 0, float *warm = nullptr)
+template <bool WARM = false, class Con>
+__device__ inline void solve_team(const Team &t, const Planes &B, int n, const Con &con, int ncon, int *done,
+                                  int *head, const FloorDev &fl, const StepConsts &k, float Cf, float Cr,
+                                  ContactTerms *pre = nullptr, int npre = 0, float *warm = nullptr)
 {
-    const int tid = threadIdx.x;
+    const int tid = t.rank, BLOCK = t.size;
     npre = min(npre, ncon);
     for (int c = tid; c < npre; c += BLOCK) contact_terms(B, con(c), fl, k, pre[c]);
     // a thread's first SOLVE_REG contacts live in registers: their bodies, and
@@ -220,13 +230,13 @@ __device__ inline void solve_rounds(const Planes &B, int n, const Con &con, int 
     // needs clearing only when the round number wraps (every 2047 rounds).
     unsigned *hk = reinterpret_cast<unsigned *>(head);
     for (int i = tid; i < n; i += BLOCK) hk[i] = 0u;
-    __syncthreads();
+    t.sync();
     int pending = ncon;
     unsigned round = 0;
     while (pending > 0) {
         if (++round == 2048u) {
             for (int i = tid; i < n; i += BLOCK) hk[i] = 0u;
-            __syncthreads();
+            t.sync();
             round = 1;
         }
         const unsigned rk = (round << 21) | 0x1FFFFFu;
@@ -246,7 +256,7 @@ __device__ inline void solve_rounds(const Planes &B, int n, const Con &con, int 
                 if (b >= 0) atomicMax(&hk[b], key(c));
             }
         }
-        __syncthreads();
+        t.sync();
         int left = 0;
         auto apply = [&](int c) {
             ContactRec r = con(c);
@@ -293,7 +303,7 @@ __device__ inline void solve_rounds(const Planes &B, int n, const Con &con, int 
                 }
             }
         }
-        pending = __syncthreads_count(left);
+        pending = t.sync_count(left);
     }
 #pragma unroll
     for (int r = 0; r < SOLVE_REG; ++r) {
@@ -304,6 +314,14 @@ __device__ inline void solve_rounds(const Planes &B, int n, const Con &con, int 
             con(c).dlt2 = dl[r][2];
         }
     }
+}
+
+template <int BLOCK, bool WARM = false, class Con>
+__device__ inline void solve_rounds(const Planes &B, int n, const Con &con, int ncon, int *done, int *head,
+                                    const FloorDev &fl, const StepConsts &k, float Cf, float Cr,
+                                    ContactTerms *pre = nullptr, int npre = 0, float *warm = nullptr)
+{
+    solve_team<WARM>(Team::block(BLOCK), B, n, con, ncon, done, head, fl, k, Cf, Cr, pre, npre, warm);
 }
 
 // ------------------------------------------------------------------ plain path kernels
@@ -544,6 +562,253 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
 
 // ------------------------------------------------------------------ batched kernel
 
+// Floats of one world's LDS for up to rows bodies (F_NLDS planes, 24 corner
+// coordinates, the bounding radius and the solver's head per body).
+__host__ __device__ constexpr int world_lds_floats(int rows) { return (F_NLDS + 26) * rows; }
+
+// One world's LDS in the batched step: body planes, box corners, bounding radii
+// and the solver's heads, for up to rows bodies.
+struct WorldLds {
+    float *body;   // [F_NLDS][rows]
+    float *verts;  // [rows][24]
+    float *brad;   // [rows]
+    int *head;     // [rows]
+    int rows;
+};
+
+__device__ inline WorldLds world_lds(float *base, int rows)
+{
+    WorldLds L;
+    L.body = base;
+    L.verts = base + F_NLDS * rows;
+    L.brad = L.verts + 24 * rows;
+    L.head = reinterpret_cast<int *>(L.brad + rows);
+    L.rows = rows;
+    return L;
+}
+
+// One world's global scratch, cap entries each: candidates, their contact records
+// and flags (the flags become the solver's done marks), the EPA queue (whose
+// simplex memory then holds the ordered contact list).
+struct WorldScratch {
+    int *cand;
+    ContactRec *rec;
+    int *flag;
+    int *qidx;
+    SimplexRec *qsim;
+    int cap;
+};
+
+// The whole step of world w by team t (the workgroup, or one wave of it): epa[k]
+// is the polytope LDS of the team's wave k, wave_tot one int per wave.
+template <int BLOCK, bool WARM>
+__device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const Team &t, int w, const WorldLds &L,
+                                              const WorldScratch &S, EpaLds *epa, NarrowShared &nsh,
+                                              int *wave_tot, const FloorDev &fl)
+{
+    constexpr int NWMAX = BLOCK / 64;
+    const int tid = t.rank, T = t.size, NW = t.waves();
+    const int b0 = a.offsets[w];
+    const int n = a.offsets[w + 1] - b0;
+    const int nb = a.nboxes[w];
+    const int ns = n - nb;
+    const Planes Lb{L.body, L.rows};
+    const Planes Gw{a.planes + b0, (int)a.N};
+    float *const vertsL = L.verts, *const bradL = L.brad;
+    int *cand = S.cand;
+    ContactRec *rec = S.rec;
+    int *flag = S.flag;
+    int *done = flag;
+    ContactRec *con = (ContactRec *)S.qsim;
+    // load the world, IntegrateForces, corners
+    for (int i = tid; i < n; i += T) {
+#pragma unroll
+        for (int f = 0; f < F_NLDS; ++f) Lb.at(f, i) = Gw.at(f, i);
+        V3 F = Gw.get3(F_FX, i), Tq = Gw.get3(F_TX, i);
+        integrate_forces_body(Lb, i, F, Tq, a.k);
+        Gw.set3(F_FX, i, mk3(0.0f, 0.0f, 0.0f));
+        Gw.set3(F_TX, i, mk3(0.0f, 0.0f, 0.0f));
+        bradL[i] = i >= nb ? Lb.at(F_SX, i)
+                           : (a.k.quat ? box_vertices_quat(Lb, Gw, i, vertsL + 24 * i)
+                                       : box_vertices_body(Lb, i, vertsL + 24 * i));
+    }
+    t.sync();
+    // cull over the canonical pairs, in canonical order: wave w takes the w-th
+    // contiguous share of the list, with no workgroup barrier, and appends the
+    // survivors to its own region of the contact-record scratch (idle until the
+    // narrowphase); the regions are then concatenated into cand[] in wave order
+    // (a one-wave team writes cand[] directly)
+    const int64_t npairs = pair_count(nb, ns);
+    const int wv = tid >> 6, lane = tid & 63;
+    const int64_t share = (npairs + NW - 1) / NW;
+    const int64_t q0 = min(npairs, (int64_t)wv * share), q1 = min(npairs, q0 + share);
+    int *wbuf = NW == 1 ? cand : reinterpret_cast<int *>(rec) + (int64_t)wv * S.cap;  // cap ints per wave
+    int mine = 0;
+    // A 64-pair chunk of box-box pairs covers the end of row ci from column cj
+    // and the rows after it: a uniform walk over those rows (one or two while
+    // rows are long) gives each lane its pair and the next chunk's start; the
+    // other kinds (after the box-box triangle) are decoded per lane.
+    const int ncc = nb * (nb - 1) / 2;
+    int ci = 0, cj = 0;
+    if (q0 < min(q1, (int64_t)ncc)) {
+        const PairId s0 = decode_pair32((int)q0, nb, ns);
+        ci = __builtin_amdgcn_readfirstlane(s0.i);  // the same in every lane: kept scalar
+        cj = __builtin_amdgcn_readfirstlane(s0.j);
+    }
+    for (int64_t b = q0; b < q1; b += 64) {
+        const int p = (int)(b + lane);
+        bool k = false;
+        if (b < ncc) {
+            int sa = 0, sb = 0, acc = 0, row = ci, col = cj;
+            for (;;) {
+                const int len = nb - col;  // pairs (row, col..nb-1)
+                if (lane >= acc && lane < acc + len) {
+                    sa = row;
+                    sb = col + lane - acc;
+                }
+                if (acc + len >= 64 || row >= nb - 2) {  // the chunk ends in this row, or the triangle does
+                    if (acc + len > 64) {
+                        ci = row;
+                        cj = col + 64 - acc;
+                    } else {
+                        ci = row + 1;
+                        cj = row + 2;
+                    }
+                    break;
+                }
+                acc += len;
+                row += 1;
+                col = row + 1;
+            }
+            if (p < q1 && p < ncc) k = cull_keep(Lb.get3(F_PX, sa), bradL[sa], Lb.get3(F_PX, sb), bradL[sb]);
+        }
+        if (p >= ncc && p < q1) {
+            const PairId q = decode_pair32(p, nb, ns);
+            int sa, sb;
+            pair_slots(q, nb, sa, sb);
+            V3 ca, cb;
+            float ra, rb;
+            slot_sphere(Lb, bradL, sa, fl, ca, ra);
+            slot_sphere(Lb, bradL, sb, fl, cb, rb);
+            k = cull_keep(ca, ra, cb, rb);
+        }
+        const uint64_t m = ballot(k);
+        const int r = mine + __popcll(m & ((1ull << lane) - 1ull));
+        if (k && r < S.cap) wbuf[r] = p;
+        mine += __popcll(m);
+    }
+    int ncand = mine;
+    if (NW > 1) {
+        if (lane == 0) wave_tot[wv] = mine;
+        t.sync();
+        ncand = 0;
+        int woff[NWMAX];
+#pragma unroll
+        for (int i = 0; i < NWMAX; ++i) {
+            woff[i] = ncand;
+            ncand += i < NW ? wave_tot[i] : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < NWMAX; ++i) {
+            if (i >= NW) break;
+            const int cnt = min(wave_tot[i], S.cap);
+            const int *src = reinterpret_cast<const int *>(rec) + (int64_t)i * S.cap;
+            for (int k = tid; k < cnt && woff[i] + k < S.cap; k += T) cand[woff[i] + k] = src[k];
+        }
+    }
+    t.sync();
+    if (ncand > S.cap) {
+        if (tid == 0) atomicAdd(&a.cnt[CNT_CAND_OVERFLOW], 1ull);
+        ncand = S.cap;
+    }
+    if (tid == 0) {
+        atomicAdd(&a.cnt[CNT_PAIR_TESTS], (unsigned long long)npairs);
+        atomicAdd(&a.cnt[CNT_CANDIDATES], (unsigned long long)ncand);
+    }
+    t.sync();
+    // narrowphase over the candidates (GJK per lane, EPA per wave)
+    {
+        WorldSrc<true> src{Lb, vertsL, nb, ns, &fl, cand, rec, flag, a.cnt};
+        narrow_team<false>(t, src, ncand, a.k.np, epa, nsh, S.qidx, S.qsim);
+    }
+    if (tid == 0) {
+        atomicAdd(&a.cnt[CNT_GJK_FOUND], (unsigned long long)nsh.nepa);
+        if (nsh.stalled) atomicAdd(&a.cnt[CNT_EPA_OVERFLOW], (unsigned long long)nsh.stalled);
+    }
+    // the contacts in candidate (= canonical) order; con reuses the queue's memory
+    // (warm start: cand[] is compacted alongside, in place, to the contacts'
+    // canonical pair indices; team_rank's barriers order the reads before the
+    // writes, and a one-wave team reads before it writes in each pass)
+    int ncon = 0;
+    for (int base = 0; base < ncand; base += T) {
+        int c = base + tid;
+        bool hit = c < ncand && flag[c];
+        const int key = WARM && hit ? cand[c] : 0;
+        int tot;
+        int rk = team_rank(t, hit, wave_tot, tot);
+        if (hit) {
+            con[ncon + rk] = rec[c];
+            if (WARM) cand[ncon + rk] = key;
+        }
+        ncon += tot;
+    }
+    if (tid == 0) atomicAdd(&a.cnt[CNT_CONTACTS], (unsigned long long)ncon);
+    t.sync();
+    // warm start: each contact's accumulators from the previous step's list (zero
+    // when its pair was no contact there), held in rec[]'s memory, idle from here
+    float *wsum = reinterpret_cast<float *>(rec);  // [3 ncon]
+    if constexpr (WARM) {
+        const int64_t off = a.warm_off[w];
+        const int *pp = a.warm_p_prev + off;
+        const float *sp = a.warm_s_prev + 3 * off;
+        const int np = a.warm_n_prev[w];
+        for (int c = tid; c < ncon; c += T) {
+            const int key = cand[c];
+            int lo = 0, hi = np;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (pp[mid] < key) lo = mid + 1;
+                else hi = mid;
+            }
+            const bool hit = lo < np && pp[lo] == key;
+            for (int q = 0; q < 3; ++q) wsum[3 * c + q] = hit ? sp[3 * lo + q] : 0.0f;
+        }
+        t.sync();
+    }
+    const float Cf = a.wfric ? a.wfric[w] : a.k.friction;
+    const float Cr = a.wrest ? a.wrest[w] : a.k.restitution;
+    solve_team<WARM>(t, Lb, n, DirectCon{con}, ncon, done, L.head, fl, a.k, Cf, Cr,
+                     reinterpret_cast<ContactTerms *>(epa), (int)(NW * sizeof(EpaLds) / sizeof(ContactTerms)),
+                     WARM ? wsum : nullptr);
+    if constexpr (WARM) {  // this step's list: the first warm_cap[w] contacts (contact c's sums are team thread c % T's)
+        const int64_t off = a.warm_off[w];
+        const int m = min(ncon, a.warm_cap[w]);
+        for (int c = tid; c < m; c += T) {
+            a.warm_p_next[off + c] = cand[c];
+            for (int q = 0; q < 3; ++q) a.warm_s_next[3 * (off + c) + q] = wsum[3 * c + q];
+        }
+        if (tid == 0) a.warm_n_next[w] = m;
+    }
+    // IntegrateVelocities and write-back
+    for (int i = tid; i < n; i += T) {
+        if (a.k.quat) integrate_velocities_quat(Lb, Gw, i, a.k.dt);
+        else integrate_velocities_body(Lb, i, a.k.dt);
+#pragma unroll
+        for (int f = 0; f < F_SX; ++f) Gw.at(f, i) = Lb.at(f, i);
+    }
+    if (a.rec_out) {
+        ContactRec *dst = a.rec_out + a.rec_off[w];
+        int rc = a.rec_cap[w];
+        for (int c = tid; c < ncon && c < rc; c += T) dst[c] = con[c];
+        if (tid == 0) a.rec_count[w] = ncon;
+    }
+    t.sync();
+}
+
+// The persistent step: the worlds order[0, nbig) (the larger ones, largest
+// first) each by a whole workgroup, then order[nbig, nworlds) each by one wave,
+// every wave of a workgroup taking its own.  Each phase hands out its worlds
+// from its own counter (queue[0], queue[1]).
 // WARM: the opt-in warm start, each contact's accumulators carried over from the
 // world's previous step (a separate instantiation; the default step has none of it).
 template <int BLOCK, bool WARM = false>
@@ -554,207 +819,42 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
     __shared__ int wave_tot[NW];
     __shared__ int s_job;
     __shared__ FloorDev fl;
+    __shared__ NarrowShared nsh[NW];
     if (threadIdx.x == 0) fl = *a.fl;
-    const int tid = threadIdx.x;
-    const int nmax = a.nmax;
-    float *bodyL = smem;                               // [F_NLDS][nmax]
-    float *vertsL = bodyL + F_NLDS * nmax;             // [nmax][24]
-    float *bradL = vertsL + 24 * nmax;                 // [nmax]
-    int *headL = (int *)(bradL + nmax);                // [nmax]
-    EpaLds *epa = (EpaLds *)(headL + nmax + ((4 - (nmax & 3)) & 3));
-    __shared__ NarrowShared nsh;
-    // per-workgroup scratch, a.cap entries each: candidates, their contact records
-    // and flags (the flags become the solver's done marks), the EPA queue (whose
-    // simplex memory then holds the ordered contact list)
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    EpaLds *const epa = reinterpret_cast<EpaLds *>(smem + a.lds_floats);
     const int64_t wg = (int64_t)blockIdx.x * a.cap;
-    int *cand = a.cand + wg;
-    ContactRec *rec = a.rec + wg;
-    int *flag = a.flag + wg;
-    int *done = flag;
-    int *qidx = a.qidx + wg;
-    SimplexRec *qsim = a.qsim + wg;
-    ContactRec *con = (ContactRec *)qsim;
-
+    // the workgroup's team, LDS and scratch until the first phase ends
+    Team t = Team::block(BLOCK);
+    WorldLds L = world_lds(smem, a.nmax);
+    WorldScratch S{a.cand + wg, a.rec + wg, a.flag + wg, a.qidx + wg, a.qsim + wg, a.cap};
+    EpaLds *tepa = epa;
+    int tsh = 0;
+    bool whole = true;
     for (;;) {
-        if (tid == 0) s_job = atomicAdd(a.queue, 1);
-        __syncthreads();
-        const int job = s_job;
-        __syncthreads();
-        if (job >= a.nworlds) break;
-        const int w = a.order[job];
-        const int b0 = a.offsets[w];
-        const int n = a.offsets[w + 1] - b0;
-        const int nb = a.nboxes[w];
-        const int ns = n - nb;
-        const Planes Lb{bodyL, nmax};
-        const Planes Gw{a.planes + b0, (int)a.N};
-        // load the world, IntegrateForces, corners
-        for (int i = tid; i < n; i += BLOCK) {
-#pragma unroll
-            for (int f = 0; f < F_NLDS; ++f) Lb.at(f, i) = Gw.at(f, i);
-            V3 F = Gw.get3(F_FX, i), T = Gw.get3(F_TX, i);
-            integrate_forces_body(Lb, i, F, T, a.k);
-            Gw.set3(F_FX, i, mk3(0.0f, 0.0f, 0.0f));
-            Gw.set3(F_TX, i, mk3(0.0f, 0.0f, 0.0f));
-            bradL[i] = i >= nb ? Lb.at(F_SX, i)
-                               : (a.k.quat ? box_vertices_quat(Lb, Gw, i, vertsL + 24 * i)
-                                           : box_vertices_body(Lb, i, vertsL + 24 * i));
-        }
-        __syncthreads();
-        // cull over the canonical pairs, in canonical order: wave w takes the w-th
-        // contiguous share of the list, with no workgroup barrier, and appends the
-        // survivors to its own region of the contact-record scratch (idle until the
-        // narrowphase); the regions are then concatenated into cand[] in wave order
-        const int64_t npairs = pair_count(nb, ns);
-        const int wv = tid >> 6, lane = tid & 63;
-        const int64_t share = (npairs + NW - 1) / NW;
-        const int64_t q0 = min(npairs, (int64_t)wv * share), q1 = min(npairs, q0 + share);
-        int *wbuf = reinterpret_cast<int *>(rec) + (int64_t)wv * a.cap;  // cap ints per wave
-        int mine = 0;
-        // While a 64-pair chunk lies in two box rows of 64 or more pairs, its pairs
-        // follow from the chunk's start (row ci, column cj), advanced by 64 per
-        // chunk; elsewhere (short rows at the triangle's end, the other kinds) each
-        // lane decodes its own index.
-        const int ncc = nb * (nb - 1) / 2;
-        bool fast = q0 < ncc;
-        int ci = 0, cj = 0;
-        if (fast) {
-            const PairId s0 = decode_pair32((int)q0, nb, ns);
-            ci = s0.i;
-            cj = s0.j;
-        }
-        for (int64_t b = q0; b < q1; b += 64) {
-            const int p = (int)(b + lane);
-            bool k = false;
-            fast = fast && nb - ci - 2 >= 64 && b + 64 <= ncc;
-            if (fast) {
-                if (p < q1) {
-                    const int jj = cj + lane;
-                    const int sa = jj < nb ? ci : ci + 1;
-                    const int sb = jj < nb ? jj : ci + 2 + (jj - nb);
-                    k = cull_keep(Lb.get3(F_PX, sa), bradL[sa], Lb.get3(F_PX, sb), bradL[sb]);
-                }
-                cj += 64;
-                if (cj >= nb) {
-                    cj = ci + 2 + (cj - nb);
-                    ci += 1;
-                }
-            } else if (p < q1) {
-                const PairId q = decode_pair32(p, nb, ns);
-                int sa, sb;
-                pair_slots(q, nb, sa, sb);
-                V3 ca, cb;
-                float ra, rb;
-                slot_sphere(Lb, bradL, sa, fl, ca, ra);
-                slot_sphere(Lb, bradL, sb, fl, cb, rb);
-                k = cull_keep(ca, ra, cb, rb);
-            }
-            const uint64_t m = ballot(k);
-            const int r = mine + __popcll(m & ((1ull << lane) - 1ull));
-            if (k && r < a.cap) wbuf[r] = p;
-            mine += __popcll(m);
-        }
-        if (lane == 0) wave_tot[wv] = mine;
-        __syncthreads();
-        int ncand = 0;
-        int woff[NW];
-#pragma unroll
-        for (int i = 0; i < NW; ++i) {
-            woff[i] = ncand;
-            ncand += wave_tot[i];
-        }
-#pragma unroll
-        for (int i = 0; i < NW; ++i) {
-            const int cnt = min(wave_tot[i], a.cap);
-            const int *src = reinterpret_cast<const int *>(rec) + (int64_t)i * a.cap;
-            for (int k = tid; k < cnt && woff[i] + k < a.cap; k += BLOCK) cand[woff[i] + k] = src[k];
-        }
-        __syncthreads();
-        if (ncand > a.cap) {
-            if (tid == 0) atomicAdd(&a.cnt[CNT_CAND_OVERFLOW], 1ull);
-            ncand = a.cap;
-        }
-        if (tid == 0) {
-            atomicAdd(&a.cnt[CNT_PAIR_TESTS], (unsigned long long)npairs);
-            atomicAdd(&a.cnt[CNT_CANDIDATES], (unsigned long long)ncand);
-        }
-        __syncthreads();
-        // narrowphase over the candidates (GJK per lane, EPA per wave)
-        {
-            WorldSrc<true> src{Lb, vertsL, nb, ns, &fl, cand, rec, flag, a.cnt};
-            narrow_block<BLOCK, false>(src, ncand, a.k.np, epa, nsh, qidx, qsim);
-        }
-        if (tid == 0) {
-            atomicAdd(&a.cnt[CNT_GJK_FOUND], (unsigned long long)nsh.nepa);
-            if (nsh.stalled) atomicAdd(&a.cnt[CNT_EPA_OVERFLOW], (unsigned long long)nsh.stalled);
-        }
-        // the contacts in candidate (= canonical) order; con reuses the queue's memory
-        // (warm start: cand[] is compacted alongside, in place, to the contacts'
-        // canonical pair indices; block_rank's barriers order the reads before the writes)
-        int ncon = 0;
-        for (int base = 0; base < ncand; base += BLOCK) {
-            int c = base + tid;
-            bool hit = c < ncand && flag[c];
-            const int key = WARM && hit ? cand[c] : 0;
-            int tot;
-            int rk = block_rank<BLOCK>(hit, wave_tot, tot);
-            if (hit) {
-                con[ncon + rk] = rec[c];
-                if (WARM) cand[ncon + rk] = key;
-            }
-            ncon += tot;
-        }
-        if (tid == 0) atomicAdd(&a.cnt[CNT_CONTACTS], (unsigned long long)ncon);
-        __syncthreads();
-        // warm start: each contact's accumulators from the previous step's list (zero
-        // when its pair was no contact there), held in rec[]'s memory, idle from here
-        float *wsum = reinterpret_cast<float *>(rec);  // [3 ncon]
-        if constexpr (WARM) {
-            const int64_t off = a.warm_off[w];
-            const int *pp = a.warm_p_prev + off;
-            const float *sp = a.warm_s_prev + 3 * off;
-            const int np = a.warm_n_prev[w];
-            for (int c = tid; c < ncon; c += BLOCK) {
-                const int key = cand[c];
-                int lo = 0, hi = np;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (pp[mid] < key) lo = mid + 1;
-                    else hi = mid;
-                }
-                const bool hit = lo < np && pp[lo] == key;
-                for (int q = 0; q < 3; ++q) wsum[3 * c + q] = hit ? sp[3 * lo + q] : 0.0f;
-            }
+        int job = 0;
+        if (whole) {
+            if (threadIdx.x == 0) s_job = atomicAdd(&a.queue[0], 1);
             __syncthreads();
-        }
-        const float Cf = a.wfric ? a.wfric[w] : a.k.friction;
-        const float Cr = a.wrest ? a.wrest[w] : a.k.restitution;
-        solve_rounds<BLOCK, WARM>(Lb, n, DirectCon{con}, ncon, done, headL, fl, a.k, Cf, Cr,
-                                  reinterpret_cast<ContactTerms *>(epa),
-                                  (int)(NW * sizeof(EpaLds) / sizeof(ContactTerms)), WARM ? wsum : nullptr);
-        if constexpr (WARM) {  // this step's list: the first warm_cap[w] contacts (contact c's sums are thread c % BLOCK's)
-            const int64_t off = a.warm_off[w];
-            const int m = min(ncon, a.warm_cap[w]);
-            for (int c = tid; c < m; c += BLOCK) {
-                a.warm_p_next[off + c] = cand[c];
-                for (int q = 0; q < 3; ++q) a.warm_s_next[3 * (off + c) + q] = wsum[3 * c + q];
+            job = __builtin_amdgcn_readfirstlane(s_job);
+            __syncthreads();
+            if (job >= a.nbig) {  // the wave phase: each wave its own LDS part and scratch
+                whole = false;
+                t = Team::wave();
+                L = world_lds(smem + wv * world_lds_floats(a.wrows), a.wrows);
+                const int64_t o = wg + (int64_t)wv * a.wcap;
+                S = WorldScratch{a.cand + o, a.rec + o, a.flag + o, a.qidx + o, a.qsim + o, a.wcap};
+                tepa = epa + wv;
+                tsh = wv;
             }
-            if (tid == 0) a.warm_n_next[w] = m;
         }
-        // IntegrateVelocities and write-back
-        for (int i = tid; i < n; i += BLOCK) {
-            if (a.k.quat) integrate_velocities_quat(Lb, Gw, i, a.k.dt);
-            else integrate_velocities_body(Lb, i, a.k.dt);
-#pragma unroll
-            for (int f = 0; f < F_SX; ++f) Gw.at(f, i) = Lb.at(f, i);
+        if (!whole) {
+            int j = 0;
+            if ((threadIdx.x & 63) == 0) j = atomicAdd(&a.queue[1], 1);
+            job = a.nbig + __builtin_amdgcn_readfirstlane(j);
         }
-        if (a.rec_out) {
-            ContactRec *dst = a.rec_out + a.rec_off[w];
-            int rc = a.rec_cap[w];
-            for (int c = tid; c < ncon && c < rc; c += BLOCK) dst[c] = con[c];
-            if (tid == 0) a.rec_count[w] = ncon;
-        }
-        __syncthreads();
+        if (job >= a.nworlds) break;
+        batched_world<BLOCK, WARM>(a, t, a.order[job], L, S, tepa, nsh[tsh], wave_tot, fl);
     }
 }
 
@@ -907,11 +1007,57 @@ static int alloc_recording(nans_ctx *ctx)
     return NANS_OK;
 }
 
-static size_t batch_lds_bytes(int nmax, int block)
+// How the batched step splits the worlds between whole workgroups and single
+// waves when worlds of at most `bodies` bodies run one per wave.
+struct WaveSplit {
+    int bodies;  // the bound (0: every world by a workgroup)
+    int nbig;    // worlds run by a workgroup: the first nbig of the order (largest first)
+    int nwave;   // worlds run one per wave
+    int wrows;   // largest of those
+    int64_t wpairs;  // their largest pair count
+};
+
+static WaveSplit wave_split(const nans_worlds *W, int bodies)
 {
-    int pad = (4 - (nmax & 3)) & 3;
-    return sizeof(float) * ((size_t)(F_NLDS + 24 + 1) * nmax) + sizeof(int) * (size_t)(nmax + pad) +
-           sizeof(EpaLds) * (size_t)(block / 64);
+    WaveSplit sp{bodies, 0, 0, 0, 0};
+    for (int w = 0; w < W->nworlds; ++w) {
+        const int n = W->h_off[w + 1] - W->h_off[w];
+        if (n > bodies) {
+            ++sp.nbig;
+        } else {
+            ++sp.nwave;
+            sp.wrows = std::max(sp.wrows, n);
+            sp.wpairs = std::max(sp.wpairs, pair_count(W->h_nb[w], n - W->h_nb[w]));
+        }
+    }
+    return sp;
+}
+
+// Floats of world LDS ahead of the polytope LDS: a workgroup's world, or one world
+// per wave, whichever is larger, rounded up to 16 bytes.
+static int batch_lds_floats(int nmax, const WaveSplit &sp, int block)
+{
+    int f = std::max(world_lds_floats(nmax), sp.nwave ? (block / 64) * world_lds_floats(sp.wrows) : 0);
+    return (f + 3) & ~3;
+}
+
+static size_t batch_lds_bytes(int nmax, const WaveSplit &sp, int block)
+{
+    return sizeof(float) * (size_t)batch_lds_floats(nmax, sp, block) + sizeof(EpaLds) * (size_t)(block / 64);
+}
+
+// Default bound of the worlds run one per wave: none.  A world on one wave takes
+// 1.5x (8-15 bodies) to 2.9x (32-63 bodies) as long as on a 256-thread
+// workgroup, so the wave phase, which runs last, lengthens the step's tail more
+// than it saves (config 3 on the MI355X: 1.93 ms per step at bound 0, 1.92 at
+// 16, 2.12 at 64); the autotuner tries the bounds per batch.
+constexpr int WAVE_WORLD_BODIES = 0;
+
+// The wave-world bound at a workgroup size: none at one wave per workgroup.
+static int wave_bodies(const nans_worlds *W, int block)
+{
+    if (block <= 64) return 0;
+    return W->tuned_wave >= 0 ? W->tuned_wave : WAVE_WORLD_BODIES;
 }
 
 }  // namespace nans
@@ -1386,10 +1532,10 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
 namespace nans {
 
 template <int BLOCK, bool WARM = false>
-static int batched_config(nans_ctx *ctx, int *grid, size_t *lds)
+static int batched_config(nans_ctx *ctx, const WaveSplit &sp, int *grid, size_t *lds)
 {
     nans_worlds *W = ctx->worlds;
-    *lds = batch_lds_bytes(W->nmax, BLOCK);
+    *lds = batch_lds_bytes(W->nmax, sp, BLOCK);
     NANS_HIP(hipFuncSetAttribute((const void *)k_batched_step<BLOCK, WARM>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)*lds));
     int per_cu = 0;
@@ -1398,20 +1544,27 @@ static int batched_config(nans_ctx *ctx, int *grid, size_t *lds)
                  W->nmax);
     int cus = 0;
     NANS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->p.device));
-    *grid = std::max(1, std::min(W->nworlds, per_cu * cus));
+    // no more workgroups than jobs: one per workgroup world, one per NW wave worlds
+    const int jobs = sp.nbig + (sp.nwave + BLOCK / 64 - 1) / (BLOCK / 64);
+    *grid = std::max(1, std::min(jobs, per_cu * cus));
     return NANS_OK;
 }
 
 // Launch shape, per-workgroup scratch and kernel arguments of the batched step.
 template <int BLOCK, bool WARM = false>
-static int batched_prepare(nans_ctx *ctx, BatchArgs &a, int *grid_out, size_t *lds_out)
+static int batched_prepare(nans_ctx *ctx, BatchArgs &a, int *grid_out, size_t *lds_out,
+                           WaveSplit *split_out = nullptr)
 {
     nans_worlds *W = ctx->worlds;
     int grid = 0;
     size_t lds = 0;
-    NANS_TRY((batched_config<BLOCK, WARM>(ctx, &grid, &lds)));
-    int cap = (int)std::min<int64_t>(W->max_pairs, 32LL * W->nmax + 1024);
-    cap = std::max(cap, 64);
+    const WaveSplit sp = wave_split(W, wave_bodies(W, BLOCK));
+    NANS_TRY((batched_config<BLOCK, WARM>(ctx, sp, &grid, &lds)));
+    constexpr int NW = BLOCK / 64;
+    int64_t cap = std::min<int64_t>(W->max_pairs, 32LL * W->nmax + 1024);
+    cap = std::max<int64_t>(cap, 64);
+    // in the wave phase each wave has cap / NW entries: room for every pair of its world
+    if (sp.nwave) cap = std::max<int64_t>(cap, NW * std::max<int64_t>(sp.wpairs, 64));
     if (grid > W->b_grid || cap != W->b_cap) {
         release_batch_scratch(W);
         NANS_TRY(dmalloc(&W->b_cand, (size_t)grid * cap));
@@ -1420,11 +1573,14 @@ static int batched_prepare(nans_ctx *ctx, BatchArgs &a, int *grid_out, size_t *l
         NANS_TRY(dmalloc(&W->b_qidx, (size_t)grid * cap));
         NANS_TRY(dmalloc(&W->b_qsim, (size_t)grid * cap));
         W->b_grid = grid;
-        W->b_cap = cap;
+        W->b_cap = (int)cap;
     }
-    if (W->queue_len < 1) {
-        NANS_TRY(dmalloc(&W->queue, 1));
-        W->queue_len = 1;
+    if (W->queue_len < 2) {
+        drop_graph(W);
+        wfree(W->queue);
+        W->queue = nullptr;
+        NANS_TRY(dmalloc(&W->queue, 2));
+        W->queue_len = 2;
     }
     a.planes = W->planes;
     a.N = W->N;
@@ -1432,7 +1588,10 @@ static int batched_prepare(nans_ctx *ctx, BatchArgs &a, int *grid_out, size_t *l
     a.nboxes = W->nboxes;
     a.order = W->order;
     a.nworlds = W->nworlds;
+    a.nbig = sp.nbig;
     a.nmax = W->nmax;
+    a.wrows = sp.wrows;
+    a.lds_floats = batch_lds_floats(W->nmax, sp, BLOCK);
     a.k = step_consts(ctx->p, ctx->orientation);
     a.fl = W->floor;
     a.wfric = W->wfric;
@@ -1443,12 +1602,14 @@ static int batched_prepare(nans_ctx *ctx, BatchArgs &a, int *grid_out, size_t *l
     a.qidx = W->b_qidx;
     a.qsim = W->b_qsim;
     a.cap = W->b_cap;
+    a.wcap = W->b_cap / NW;
     a.rec_out = W->record ? W->rec_buf : nullptr;
     a.rec_off = W->rec_off;
     a.rec_count = W->rec_count;
     a.rec_cap = W->rec_cap;
     a.cnt = W->cnt;
     a.queue = W->queue;
+    if (split_out) *split_out = sp;
     *grid_out = grid;
     *lds_out = lds;
     return NANS_OK;
@@ -1461,12 +1622,13 @@ static int batched_capture(nans_ctx *ctx, int nsteps, hipStream_t cs)
     BatchArgsT<false> a;
     int grid = 0;
     size_t lds = 0;
-    NANS_TRY(batched_prepare<BLOCK>(ctx, a, &grid, &lds));
-    // one step: reset the work counter with a fill kernel (no memset node), then the
+    WaveSplit sp;
+    NANS_TRY(batched_prepare<BLOCK>(ctx, a, &grid, &lds, &sp));
+    // one step: reset the work counters with a fill kernel (no memset node), then the
     // persistent step kernel; a call of nsteps replays this graph nsteps times
     hipGraph_t g = nullptr;
     NANS_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    k_zero_i32<<<1, 64, 0, cs>>>(W->queue, 1);
+    k_zero_i32<<<1, 64, 0, cs>>>(W->queue, 2);
     k_batched_step<BLOCK><<<grid, BLOCK, lds, cs>>>(a);
     hipError_t e = hipStreamEndCapture(cs, &g);
     NANS_REQUIRE(e == hipSuccess, NANS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
@@ -1480,6 +1642,8 @@ static int batched_capture(nans_ctx *ctx, int nsteps, hipStream_t cs)
     W->graph_orientation = ctx->orientation;
     W->graph_grid = grid;
     W->graph_lds = (int)lds;
+    W->graph_wave = sp.bodies;
+    W->graph_nwave = sp.nwave;
     return NANS_OK;
 }
 
@@ -1496,7 +1660,8 @@ int batched_run(nans_ctx *ctx, int nsteps, int block)
         NANS_HIP(hipEventCreate(&W->ev_t0));
         NANS_HIP(hipEventCreate(&W->ev_t1));
     }
-    bool reuse = W->graph && W->graph_block == block && W->graph_orientation == ctx->orientation &&
+    bool reuse = W->graph && W->graph_block == block && W->graph_wave == wave_bodies(W, block) &&
+                 W->graph_orientation == ctx->orientation &&
                  memcmp(&W->graph_params, &ctx->p, sizeof(nans_params)) == 0;
     if (!reuse) {
         NANS_TRY(setup_floor(ctx, ctx->stream));
@@ -1529,7 +1694,8 @@ static int batched_warm_launches(nans_ctx *ctx, int nsteps, hipStream_t cs)
     BatchArgsT<true> a;
     int grid = 0;
     size_t lds = 0;
-    NANS_TRY((batched_prepare<BLOCK, true>(ctx, a, &grid, &lds)));
+    WaveSplit sp;
+    NANS_TRY((batched_prepare<BLOCK, true>(ctx, a, &grid, &lds, &sp)));
     a.warm_off = W->warm_offd;
     a.warm_cap = W->warm_capd;
     for (int s = 0; s < nsteps; ++s) {
@@ -1540,7 +1706,7 @@ static int batched_warm_launches(nans_ctx *ctx, int nsteps, hipStream_t cs)
         a.warm_p_next = W->warm_p[nxt];
         a.warm_s_next = W->warm_s[nxt];
         a.warm_n_next = W->warm_n[nxt];
-        k_zero_i32<<<1, 64, 0, cs>>>(a.queue, 1);
+        k_zero_i32<<<1, 64, 0, cs>>>(a.queue, 2);
         k_batched_step<BLOCK, true><<<grid, BLOCK, lds, cs>>>(a);
         NANS_HIP(hipGetLastError());
         W->warm_flip = nxt;
@@ -1548,6 +1714,8 @@ static int batched_warm_launches(nans_ctx *ctx, int nsteps, hipStream_t cs)
     W->graph_block = BLOCK;
     W->graph_grid = grid;
     W->graph_lds = (int)lds;
+    W->graph_wave = sp.bodies;
+    W->graph_nwave = sp.nwave;
     return NANS_OK;
 }
 
@@ -1605,7 +1773,7 @@ static int batched_recorded_launch(nans_ctx *ctx, hipStream_t s, ContactRec *rec
     a.rec_off = rec_off;
     a.rec_count = rec_count;
     a.rec_cap = rec_cap;
-    k_zero_i32<<<1, 64, 0, s>>>(a.queue, 1);
+    k_zero_i32<<<1, 64, 0, s>>>(a.queue, 2);
     k_batched_step<BLOCK><<<grid, BLOCK, lds, s>>>(a);
     NANS_HIP(hipGetLastError());
     return NANS_OK;
@@ -1659,6 +1827,8 @@ int nans_batched_timing(nans_ctx *ctx, nans_timing *t)
     t->threads_per_world = W->graph_block;
     t->workgroups = W->graph_grid;
     t->lds_bytes = W->graph_lds;
+    t->wave_world_bodies = W->graph_wave;
+    t->wave_worlds = W->graph_nwave;
     return NANS_OK;
 }
 
@@ -1669,6 +1839,15 @@ int nans_set_batched_block(nans_ctx *ctx, int32_t threads_per_world)
                      threads_per_world == 256 || threads_per_world == 512,
                  NANS_ERR_INVALID, "threads_per_world must be 0, 64, 128, 256 or 512 (got %d)", threads_per_world);
     ctx->worlds->tuned_block = threads_per_world;
+    return NANS_OK;
+}
+
+int nans_set_wave_worlds(nans_ctx *ctx, int32_t max_bodies)
+{
+    NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
+    NANS_REQUIRE(max_bodies >= -1 && max_bodies <= BATCH_MAX_BODIES, NANS_ERR_INVALID,
+                 "max_bodies must be -1 (default), 0 (none) or at most %d (got %d)", BATCH_MAX_BODIES, max_bodies);
+    ctx->worlds->tuned_wave = max_bodies;
     return NANS_OK;
 }
 
@@ -1696,45 +1875,65 @@ int nans_autotune_batched(nans_ctx *ctx, int32_t nsteps, nans_tuning *out)
     NANS_HIP(hipMemcpyAsync(keep + NSTATE * N, W->planes + (size_t)F_FX * N, sizeof(float) * NTAIL * N,
                             hipMemcpyDeviceToDevice, s));
     const int cand[4] = {64, 128, 256, 512};
-    int best = 0;
+    const int bounds[5] = {0, 16, 32, 64, 128};
+    int best = 0, best_wave = 0;
     float best_ms = 0.0f;
     int rc = NANS_OK;
     int max_lds = 0;
     NANS_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->p.device));
-    for (int c = 0; c < 4 && rc == NANS_OK; ++c) {
-        float ms = -1.0f;
-        if (batch_lds_bytes(W->nmax, cand[c]) > (size_t)max_lds) {  // does not fit: skipped
-            if (out) out->ms_per_step[c] = ms;
-            continue;
+    const int keep_wave = W->tuned_wave;
+    if (out) {
+        for (int c = 0; c < 4; ++c) {
+            out->ms_per_step[c] = -1.0f;
+            for (int b = 0; b < 5; ++b) out->ms_per_step_wave[c][b] = -1.0f;
         }
-        // one untimed step captures the graph; then nsteps timed steps
-        int r = batched_run(ctx, 1, cand[c]);
-        if (r == NANS_OK) r = batched_run(ctx, nsteps, cand[c]);
-        if (r == NANS_OK) {
-            NANS_HIP(hipStreamSynchronize(s));
-            NANS_HIP(hipEventElapsedTime(&ms, W->ev_t0, W->ev_t1));
-            ms /= (float)nsteps;
-            if (!best || ms < best_ms) {
-                best = cand[c];
-                best_ms = ms;
-            }
-        } else {
-            rc = r;
-        }
-        if (out) out->ms_per_step[c] = ms;
-        NANS_HIP(hipMemcpyAsync(W->planes, keep, sizeof(float) * NSTATE * N, hipMemcpyDeviceToDevice, s));
-        NANS_HIP(hipMemcpyAsync(W->planes + (size_t)F_FX * N, keep + NSTATE * N, sizeof(float) * NTAIL * N,
-                                hipMemcpyDeviceToDevice, s));
     }
+    for (int c = 0; c < 4 && rc == NANS_OK; ++c) {
+        int prev_nbig = -1;
+        for (int b = 0; b < 5 && rc == NANS_OK; ++b) {
+            if (cand[c] == 64 && b > 0) break;  // one wave per workgroup: no wave phase
+            const WaveSplit sp = wave_split(W, bounds[b]);
+            // the same split as the previous bound, or LDS that does not fit: skipped
+            if (sp.nbig == prev_nbig || batch_lds_bytes(W->nmax, sp, cand[c]) > (size_t)max_lds) continue;
+            prev_nbig = sp.nbig;
+            W->tuned_wave = bounds[b];
+            float ms = -1.0f;
+            // one untimed step captures the graph; then nsteps timed steps
+            int r = batched_run(ctx, 1, cand[c]);
+            if (r == NANS_OK) r = batched_run(ctx, nsteps, cand[c]);
+            if (r == NANS_OK) {
+                NANS_HIP(hipStreamSynchronize(s));
+                NANS_HIP(hipEventElapsedTime(&ms, W->ev_t0, W->ev_t1));
+                ms /= (float)nsteps;
+                if (!best || ms < best_ms) {
+                    best = cand[c];
+                    best_wave = bounds[b];
+                    best_ms = ms;
+                }
+                if (out) {
+                    out->ms_per_step_wave[c][b] = ms;
+                    if (out->ms_per_step[c] < 0.0f || ms < out->ms_per_step[c]) out->ms_per_step[c] = ms;
+                }
+            } else {
+                rc = r;
+            }
+            NANS_HIP(hipMemcpyAsync(W->planes, keep, sizeof(float) * NSTATE * N, hipMemcpyDeviceToDevice, s));
+            NANS_HIP(hipMemcpyAsync(W->planes + (size_t)F_FX * N, keep + NSTATE * N, sizeof(float) * NTAIL * N,
+                                    hipMemcpyDeviceToDevice, s));
+        }
+    }
+    W->tuned_wave = keep_wave;
     NANS_HIP(hipStreamSynchronize(s));
     wfree(keep);
     NANS_HIP(hipMemcpy(W->cnt, cnt, sizeof(cnt), hipMemcpyHostToDevice));
     if (rc != NANS_OK) return rc;
     NANS_REQUIRE(best > 0, NANS_ERR_CAPACITY, "no workgroup size fits a world of %d bodies", W->nmax);
     W->tuned_block = best;
+    W->tuned_wave = best_wave;
     W->timed_steps = 0;
     if (out) {
         out->threads_per_world = best;
+        out->wave_world_bodies = best_wave;
         out->steps = nsteps;
     }
     return NANS_OK;

@@ -68,11 +68,17 @@ class Stats(ctypes.Structure):
 
 class Timing(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int32), ("elapsed_ms", ctypes.c_float), ("threads_per_world", ctypes.c_int32),
-                ("workgroups", ctypes.c_int32), ("lds_bytes", ctypes.c_int32)]
+                ("workgroups", ctypes.c_int32), ("lds_bytes", ctypes.c_int32), ("wave_world_bodies", ctypes.c_int32),
+                ("wave_worlds", ctypes.c_int32)]
+
+
+TUNE_SIZES = (64, 128, 256, 512)
+TUNE_WAVE_BOUNDS = (0, 16, 32, 64, 128)
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("threads_per_world", ctypes.c_int32), ("steps", ctypes.c_int32), ("ms_per_step", ctypes.c_float * 4)]
+    _fields_ = [("threads_per_world", ctypes.c_int32), ("steps", ctypes.c_int32), ("ms_per_step", ctypes.c_float * 4),
+                ("wave_world_bodies", ctypes.c_int32), ("ms_per_step_wave", (ctypes.c_float * 5) * 4)]
 
 
 class AdjointIo(ctypes.Structure):
@@ -288,16 +294,24 @@ class Context:
         check(self.lib.nans_set_state(self.h, ctypes.byref(b)))
 
     def autotune(self, nsteps: int = 5) -> dict:
-        """Time the batched step at 64/128/256/512 threads per world on the uploaded
-        batch and keep the fastest (nans_autotune_batched); the state is restored."""
+        """Time the batched step at 64/128/256/512 threads per world and wave-world
+        bounds 0/16/32/64/128 bodies on the uploaded batch and keep the fastest
+        (nans_autotune_batched); the state is restored.  ms -1: skipped."""
         self._sync_stream()
         t = Tuning()
         check(self.lib.nans_autotune_batched(self.h, int(nsteps), ctypes.byref(t)))
-        return {"threads_per_world": t.threads_per_world, "steps": t.steps,
-                "ms_per_step": dict(zip((64, 128, 256, 512), [round(x, 4) for x in t.ms_per_step]))}
+        return {"threads_per_world": t.threads_per_world, "wave_world_bodies": t.wave_world_bodies,
+                "steps": t.steps,
+                "ms_per_step": dict(zip(TUNE_SIZES, [round(x, 4) for x in t.ms_per_step])),
+                "ms_per_step_wave": {sz: dict(zip(TUNE_WAVE_BOUNDS, [round(x, 4) for x in t.ms_per_step_wave[i]]))
+                                     for i, sz in enumerate(TUNE_SIZES)}}
 
     def set_batched_block(self, threads_per_world: int) -> None:
         check(self.lib.nans_set_batched_block(self.h, int(threads_per_world)))
+
+    def set_wave_worlds(self, max_bodies: int) -> None:
+        """Worlds of at most max_bodies bodies run one per wave (-1: the default, none; 0: none)."""
+        check(self.lib.nans_set_wave_worlds(self.h, int(max_bodies)))
 
     def set_warm_start(self, enable: bool) -> None:
         """Opt-in warm start of the plain and batched steps (contact accumulators carried over)."""

@@ -198,19 +198,21 @@ def test_gradients_match_fp64_autograd(c4, oracle_mod):
 
 def test_gradients_independent_of_checkpoint_spacing(c4):
     """Checkpointing changes what is recomputed, not the result: every 1, 16 and
-    240 steps give identical gradients."""
+    240 steps give identical gradients, and so does running each world's steps
+    on one wave (wave-world bound 16: all of them) instead of a workgroup."""
     sc = c4["scene"].subset(list(range(64)))
     out = []
-    for every in (1, EVERY, NSTEPS):
+    for every, bound in ((1, -1), (EVERY, -1), (NSTEPS, -1), (EVERY, 16)):
         ctx = runtime.Context(sc.params)
         ctx.upload(sc)
+        ctx.set_wave_worlds(bound)
         ctx.rollout(NSTEPS, every)
         dl = np.full((sc.nbodies, 3), 1e-3, np.float32)
         out.append(ctx.adjoint(dl_pos=dl, dl_vel=dl))
         ctx.close()
     for k in out[0]:
-        np.testing.assert_array_equal(out[1][k], out[0][k], err_msg=k)
-        np.testing.assert_array_equal(out[2][k], out[0][k], err_msg=k)
+        for o in out[1:]:
+            np.testing.assert_array_equal(o[k], out[0][k], err_msg=k)
 
 
 class Rollout(torch.autograd.Function):

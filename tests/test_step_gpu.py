@@ -95,12 +95,15 @@ def _c3_subset():
     return full.subset(sorted(set(pick)))
 
 
-@pytest.mark.parametrize("path", ["plain", "batched"])
+@pytest.mark.parametrize("path", ["plain", "batched", "batched_waves"])
 def test_config3_worlds_bit_exact(oracle_mod, path):
     """A batch of config-3 worlds (8..256 bodies, 1-4 spheres) for 40 steps:
-    every world's state and last contact list equal the oracle's."""
+    every world's state and last contact list equal the oracle's (batched_waves:
+    the worlds of at most 64 bodies one per wave)."""
     sc = _c3_subset()
     ctx = _ctx(sc)
+    if path == "batched_waves":
+        ctx.set_wave_worlds(64)
     (ctx.step_plain if path == "plain" else ctx.step_batched)(40)
     got = ctx.download()
     for w in range(sc.nworlds):
@@ -144,27 +147,42 @@ def test_plain_equals_batched_graph_replay():
     b.close()
 
 
+# (threads per world, wave-world bound): every world by a workgroup (bound 0, the
+# default -1), bounds that split the subset elsewhere, and (128, 256): every world
+# by a wave
+SPLITS = ((64, -1), (128, 0), (128, 64), (128, 256), (256, -1), (256, 16), (256, 32), (256, 64), (256, 128),
+          (512, 0), (512, 64))
+
+
 def test_workgroup_sizes_and_autotuner():
-    """Every workgroup size of the batched step gives the same bits; the autotuner
-    leaves the state as it found it and its choice steps the same."""
+    """Every workgroup size and wave-world bound of the batched step gives the same
+    bits; the autotuner leaves the state as it found it and its choice steps the
+    same."""
     sc = _c3_subset()
+    sizes = np.diff(sc.offsets)
     ref = None
-    for tpw in (64, 128, 256, 512):
+    for tpw, bound in SPLITS:
         ctx = _ctx(sc)
         ctx.set_batched_block(tpw)
+        ctx.set_wave_worlds(bound)
+        print("threads %d, wave-world bound %d" % (tpw, bound), flush=True)
         ctx.step_batched(20)
-        assert ctx.batched_timing()["threads_per_world"] == tpw
+        t = ctx.batched_timing()
+        eff = 0 if tpw == 64 or bound < 0 else bound
+        assert t["threads_per_world"] == tpw
+        assert t["wave_world_bodies"] == eff and t["wave_worlds"] == int((sizes <= eff).sum()), t
         got = ctx.download()
         ctx.close()
         if ref is None:
             ref = got
         for f in STATE:
-            np.testing.assert_array_equal(got[f], ref[f], err_msg="%d threads %s" % (tpw, f))
+            np.testing.assert_array_equal(got[f], ref[f], err_msg="%d threads, bound %d: %s" % (tpw, bound, f))
     ctx = _ctx(sc)
     before = ctx.download()
     tune = ctx.autotune(3)
     print("autotune on the config-3 subset:", tune)
     assert tune["threads_per_world"] in (64, 128, 256, 512)
+    assert tune["wave_world_bodies"] in runtime.TUNE_WAVE_BOUNDS
     after = ctx.download()
     for f in STATE:
         np.testing.assert_array_equal(after[f], before[f])

@@ -89,9 +89,13 @@ def test_set_state_empties_the_lists(oracle_mod):
     ctx.close()
 
 
-def test_config3_worlds_batched_warm(oracle_mod):
+@pytest.mark.parametrize("wave_bound", [0, 64])
+def test_config3_worlds_batched_warm(oracle_mod, wave_bound):
+    """Six config-3 worlds (8 to 249 bodies), 60 warm-started batched steps; with
+    wave_bound 64 the worlds of at most 64 bodies run one per wave."""
     sc = S.config3(128, nsteps=1).subset([0, 17, 40, 77, 102, 127])  # 8 to 249 bodies
     ctx = _warm_ctx(sc)
+    ctx.set_wave_worlds(wave_bound)
     for k in range(3):
         ctx.step_batched(20)
     got = ctx.download()
