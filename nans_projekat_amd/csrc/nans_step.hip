@@ -197,18 +197,28 @@ struct DirectCon {
     __device__ ContactRec &operator()(int c) const { return p[c]; }
 };
 
-// pre[0..npre) (may be empty) receives Constraint's position-only terms of the
-// first npre contacts in one parallel pass, so that the rounds only evaluate the
-// velocity-dependent part for them.
-// con(c) is contact c's record (DirectCon).
+// Room for Constraint's position-only terms of the first contacts: two
+// segments of LDS that are idle during SolveConstraints (terms of contact c in
+// a[c] for c < na, else in b[c - na] for c < na + nb).
+struct PreTerms {
+    ContactTerms *a = nullptr;
+    int na = 0;
+    ContactTerms *b = nullptr;
+    int nb = 0;
+    __device__ ContactTerms &operator()(int c) const { return c < na ? a[c] : b[c - na]; }
+};
+
+// pre receives the terms of the first contacts in one parallel pass (as many as
+// it holds), so that the rounds only evaluate the velocity-dependent part for
+// them.  con(c) is contact c's record (DirectCon).
 template <bool WARM = false, class Con>
 __device__ inline void solve_team(const Team &t, const Planes &B, int n, const Con &con, int ncon, int *done,
                                   int *head, const FloorDev &fl, const StepConsts &k, float Cf, float Cr,
-                                  ContactTerms *pre = nullptr, int npre = 0, float *warm = nullptr)
+                                  const PreTerms &pre = PreTerms{}, float *warm = nullptr)
 {
     const int tid = t.rank, BLOCK = t.size;
-    npre = min(npre, ncon);
-    for (int c = tid; c < npre; c += BLOCK) contact_terms(B, con(c), fl, k, pre[c]);
+    const int npre = min(pre.na + pre.nb, ncon);
+    for (int c = tid; c < npre; c += BLOCK) contact_terms(B, con(c), fl, k, pre(c));
     // a thread's first SOLVE_REG contacts live in registers: their bodies, and
     // the impulses they apply, written back once after the rounds (no global
     // memory access on a round's critical path when their terms are in pre[])
@@ -261,7 +271,7 @@ __device__ inline void solve_team(const Team &t, const Planes &B, int n, const C
         auto apply = [&](int c) {
             ContactRec r = con(c);
             float *sums = WARM ? warm + 3 * c : nullptr;
-            if (c < npre) contact_apply(B, r, pre[c], k, Cf, Cr, sums);
+            if (c < npre) contact_apply(B, r, pre(c), k, Cf, Cr, sums);
             else constraint(B, r, fl, k, Cf, Cr, sums);
             con(c).dln = r.dln;
             con(c).dlt1 = r.dlt1;
@@ -278,7 +288,7 @@ __device__ inline void solve_team(const Team &t, const Planes &B, int n, const C
                     if (c < npre) {
                         rr.slot_a = sa[r];
                         rr.slot_b = sb[r];
-                        contact_apply(B, rr, pre[c], k, Cf, Cr, sums);
+                        contact_apply(B, rr, pre(c), k, Cf, Cr, sums);
                     } else {
                         rr = con(c);
                         constraint(B, rr, fl, k, Cf, Cr, sums);
@@ -319,9 +329,9 @@ __device__ inline void solve_team(const Team &t, const Planes &B, int n, const C
 template <int BLOCK, bool WARM = false, class Con>
 __device__ inline void solve_rounds(const Planes &B, int n, const Con &con, int ncon, int *done, int *head,
                                     const FloorDev &fl, const StepConsts &k, float Cf, float Cr,
-                                    ContactTerms *pre = nullptr, int npre = 0, float *warm = nullptr)
+                                    const PreTerms &pre = PreTerms{}, float *warm = nullptr)
 {
-    solve_team<WARM>(Team::block(BLOCK), B, n, con, ncon, done, head, fl, k, Cf, Cr, pre, npre, warm);
+    solve_team<WARM>(Team::block(BLOCK), B, n, con, ncon, done, head, fl, k, Cf, Cr, pre, warm);
 }
 
 // ------------------------------------------------------------------ plain path kernels
@@ -539,7 +549,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
         __syncthreads();
     }
     solve_rounds<SOLVE_THREADS, WARM>(a.B, a.n, DirectCon{a.con}, ncon, a.done, head, fl, a.k, a.Cf, a.Cr,
-                                      nullptr, 0, a.warm_sums);
+                                      PreTerms{}, a.warm_sums);
     if (WARM) {
         __syncthreads();
         const int m = min(ncon, a.warm_cap);
@@ -566,13 +576,14 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
 // coordinates, the bounding radius and the solver's head per body).
 __host__ __device__ constexpr int world_lds_floats(int rows) { return (F_NLDS + 26) * rows; }
 
-// One world's LDS in the batched step: body planes, box corners, bounding radii
-// and the solver's heads, for up to rows bodies.
+// One world's LDS in the batched step: body planes, the solver's heads, bounding
+// radii and box corners, for up to rows bodies.  The radii and corners (idle
+// once the narrowphase ends) are adjacent: the solver keeps contact terms there.
 struct WorldLds {
     float *body;   // [F_NLDS][rows]
-    float *verts;  // [rows][24]
-    float *brad;   // [rows]
     int *head;     // [rows]
+    float *brad;   // [rows]
+    float *verts;  // [rows][24]
     int rows;
 };
 
@@ -580,9 +591,9 @@ __device__ inline WorldLds world_lds(float *base, int rows)
 {
     WorldLds L;
     L.body = base;
-    L.verts = base + F_NLDS * rows;
-    L.brad = L.verts + 24 * rows;
-    L.head = reinterpret_cast<int *>(L.brad + rows);
+    L.head = reinterpret_cast<int *>(base + F_NLDS * rows);
+    L.brad = base + (F_NLDS + 1) * rows;
+    L.verts = L.brad + rows;
     L.rows = rows;
     return L;
 }
@@ -777,9 +788,13 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
     }
     const float Cf = a.wfric ? a.wfric[w] : a.k.friction;
     const float Cr = a.wrest ? a.wrest[w] : a.k.restitution;
-    solve_team<WARM>(t, Lb, n, DirectCon{con}, ncon, done, L.head, fl, a.k, Cf, Cr,
-                     reinterpret_cast<ContactTerms *>(epa), (int)(NW * sizeof(EpaLds) / sizeof(ContactTerms)),
-                     WARM ? wsum : nullptr);
+    // contact terms in the radii and corners, then in the team's polytope LDS
+    PreTerms pre;
+    pre.a = reinterpret_cast<ContactTerms *>(L.brad);
+    pre.na = (int)(sizeof(float) * 25 * L.rows / sizeof(ContactTerms));
+    pre.b = reinterpret_cast<ContactTerms *>(epa);
+    pre.nb = (int)(NW * sizeof(EpaLds) / sizeof(ContactTerms));
+    solve_team<WARM>(t, Lb, n, DirectCon{con}, ncon, done, L.head, fl, a.k, Cf, Cr, pre, WARM ? wsum : nullptr);
     if constexpr (WARM) {  // this step's list: the first warm_cap[w] contacts (contact c's sums are team thread c % T's)
         const int64_t off = a.warm_off[w];
         const int m = min(ncon, a.warm_cap[w]);
