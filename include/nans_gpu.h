@@ -140,7 +140,7 @@ int nans_narrowphase(nans_ctx *ctx, int64_t n, const nans_shapes_soa *a, const n
  * Same arguments, results and errors as nans_narrowphase, except that at most
  * NANS_NARROWPHASE_NOALLOC_MAX_PAIRS pairs are accepted per call (more:
  * NANS_ERR_CAPACITY; split the call).  Its EPA queue lives in a static device
- * array of 25 MiB per device, held until the process ends and shared by every
+ * array of 26 MiB per device, held until the process ends and shared by every
  * call on that device: calls that overlap on one device (several threads or
  * streams at once) race on it. */
 #define NANS_NARROWPHASE_NOALLOC_MAX_PAIRS 262144

@@ -64,18 +64,19 @@ struct PairSrc {
     NarrowOutDev o;
     unsigned long long *overflow;
     static constexpr bool kMemCorners = false;
-    __device__ void shapes_ref(int c, Shape &A, Shape &B) const { shapes(c, A, B); }
-    __device__ void shapes(int c, Shape &A, Shape &B) const
+    __device__ int key(int c) const { return c; }
+    __device__ void shapes_ref(int key, Shape &A, Shape &B) const { shapes(key, A, B); }
+    __device__ void shapes(int key, Shape &A, Shape &B) const
     {
-        A = load_shape(a, base + c, n);
-        B = load_shape(b, base + c, n);
+        A = load_shape(a, base + key, n);
+        B = load_shape(b, base + key, n);
     }
     __device__ void gjk_result(int c, int g) const
     {
         const V3 z = mk3(0.0f, 0.0f, 0.0f);
         write_pair(o, base + c, n, g, EPA_NONE, z, z, z, 0.0);
     }
-    __device__ void epa_result(int c, const EpaResult &r) const
+    __device__ void epa_result(int c, int key, const EpaResult &r) const
     {
         write_pair(o, base + c, n, GJK_FOUND, r.status, r.n, r.pa, r.pb, r.depth);
         if (r.status == EPA_OVERFLOW) atomicAdd(overflow, 1ull);
@@ -112,7 +113,7 @@ __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
 // Scratch of nans_narrowphase_noalloc, one copy per device for the life of the
 // process: the EPA overflow counter (in its own 256-byte slot), then the EPA queue
 // of k_narrowphase -- a candidate index and a GJK simplex per pair -- for up to
-// NANS_NARROWPHASE_NOALLOC_MAX_PAIRS pairs (25 MiB in all).
+// NANS_NARROWPHASE_NOALLOC_MAX_PAIRS pairs (26 MiB in all).
 constexpr size_t NA_CNT_BYTES = 256;
 constexpr size_t NA_QIDX_BYTES = sizeof(int) * (size_t)NANS_NARROWPHASE_NOALLOC_MAX_PAIRS;
 constexpr size_t NA_QSIM_BYTES = sizeof(SimplexRec) * (size_t)NANS_NARROWPHASE_NOALLOC_MAX_PAIRS;

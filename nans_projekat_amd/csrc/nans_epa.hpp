@@ -1146,10 +1146,12 @@ constexpr int GJK_CYCLE_START = 8;
 // the MI355X: 1.429 ms per step at 16, 1.413 at 32-56, 1.484 at 64).
 constexpr int GJK_REFILL_MIN = 48;
 
-// A queued GJK hit's simplex: the supports on A and B of its four vertices; the
-// Minkowski point is recomputed as SupA - SupB, the expression that made it.
+// A queued GJK hit's simplex: the supports on A and B of its four vertices (the
+// Minkowski point is recomputed as SupA - SupB, the expression that made it),
+// and the pair's key (the source's key(c)), so that EPA looks nothing else up.
 struct SimplexRec {
     float a[12], b[12];
+    int key;
 };
 
 __device__ __forceinline__ void simplex_rec_put(SimplexRec &q, const Vtx &v, int k)
@@ -1179,7 +1181,7 @@ __device__ __forceinline__ void simplex_rec_store(SimplexRec &q, const Simplex &
 struct GjkLane {
     Shape A, B;
     Simplex S;
-    int c = -1, calls = 0, power = 1, lam = 0;
+    int c = -1, key = 0, calls = 0, power = 1, lam = 0;
     bool saved = false, retired = false;
 };
 
@@ -1193,7 +1195,8 @@ __device__ inline void gjk_lane_take(GjkLane &G, const Src &src, int c, int ncan
     }
     const int tid = lane_id();
     G.c = c;
-    src.shapes(c, G.A, G.B);
+    G.key = src.key(c);
+    src.shapes(G.key, G.A, G.B);
     const V3 d0 = gjk_first_dir(G.A, G.B);  // kept beside the cycle state
     cyc[13 * 64 + tid] = d0.x;
     cyc[14 * 64 + tid] = d0.y;
@@ -1238,12 +1241,13 @@ __device__ inline int gjk_lane_call(GjkLane &G, float *cyc, const NarrowParams &
 }
 
 // Source interface (duck-typed):
-//   static constexpr bool kMemCorners                  box corners readable in memory:
-//   void shapes_ref(int c, Shape &A, Shape &B) const   then both shapes of candidate c
-//                                                      with Shape::vm and no v[]
-//   void shapes(int c, Shape &A, Shape &B) const       both shapes of candidate c
-//   void gjk_result(int c, int gjk) const               every candidate
-//   void epa_result(int c, const EpaResult &r) const    every GJK hit (lane 0 calls)
+//   int key(int c) const                                 the pair of candidate c, as an int
+//   static constexpr bool kMemCorners                    box corners readable in memory:
+//   void shapes_ref(int key, Shape &A, Shape &B) const   then both shapes of the pair
+//                                                        with Shape::vm and no v[]
+//   void shapes(int key, Shape &A, Shape &B) const       both shapes of the pair
+//   void gjk_result(int c, int gjk) const                 every candidate
+//   void epa_result(int c, int key, const EpaResult &r) const   every GJK hit (lane 0 calls)
 // WIDE: polytopes that outgrow a 16-lane group go to a 32-lane group before the
 // whole wave (worth its registers where many do, as among random deep overlaps).
 // ISL: the FP64-islands mode runs the grouped tiers too (their final face choice
@@ -1287,6 +1291,7 @@ __device__ void narrow_team(const Team &team, const Src &src, int ncand, const N
                     if (g == GJK_FOUND) {
                         const int q = atomicAdd(&sh.nepa, 1);
                         simplex_rec_store(qsim[q], G.S);
+                        qsim[q].key = G.key;
                         // publish the slot after its simplex
                         __hip_atomic_store(&qidx[q], G.c, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
@@ -1307,15 +1312,28 @@ __device__ void narrow_team(const Team &team, const Src &src, int ncand, const N
         EpgLds<16> &G = *reinterpret_cast<EpgLds<16> *>(&L);
         EpgLds<32> &G2 = *reinterpret_cast<EpgLds<32> *>(&L);
         const int j = lane & 15;
-        for (;;) {
-            int base = 0;
-            if (lane == 0) base = atomicAdd(&sh.epa_next, 4);
-            base = __builtin_amdgcn_readfirstlane(base);
-            // the candidate of this group's slot, or -1 once it is certain the slot
-            // stays empty (every wave has left GJK and it was never published)
-            const int job = base + (lane >> 4);
+        // A pass takes four queue slots, one per group.  The next pass's slots are
+        // claimed while this one starts, their contents loaded (relaxed) under its
+        // work, and then confirmed by an acquire fence, or waited for.
+        auto claim = [&]() {
+            int b = 0;
+            if (lane == 0) b = atomicAdd(&sh.epa_next, 4);
+            return __builtin_amdgcn_readfirstlane(b);
+        };
+        auto peek = [&](int b) {
+            const int job = b + (lane >> 4);
+            return job < ncand ? __hip_atomic_load(&qidx[job], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : -1;
+        };
+        // the candidate of this group's slot, or -1 once it is certain the slot
+        // stays empty (every wave has left GJK and it was never published)
+        auto resolve = [&](int b, int seen) {
+            const int job = b + (lane >> 4);
             int c = -1;
             if (job < ncand) {
+                if (seen >= 0 && seen < ncand) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    return seen;
+                }
                 for (int round = 0;; ++round) {
                     c = __hip_atomic_load(&qidx[job], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
                     if (c >= ncand) {  // not a candidate of this block: never published here
@@ -1339,14 +1357,21 @@ __device__ void narrow_team(const Team &team, const Src &src, int ncand, const N
                     __builtin_amdgcn_s_sleep(2);
                 }
             }
+            return c;
+        };
+        int base = claim();
+        int c = resolve(base, peek(base));
+        for (;;) {
             // slots are filled in order, so an empty first slot ends the queue
             if (__shfl(c, 0, 64) < 0) break;
+            const int job = base + (lane >> 4);
             const bool act = c >= 0;
             const int jq = act ? job : base;
             if (!act) c = __shfl(c, 0, 64);
+            const int key = qsim[jq].key;
             Shape A, B;
-            if constexpr (Src::kMemCorners) src.shapes_ref(c, A, B);
-            else src.shapes(c, A, B);
+            if constexpr (Src::kMemCorners) src.shapes_ref(key, A, B);
+            else src.shapes(key, A, B);
             const EpaShapes es = epa_shapes_group<Src::kMemCorners>(A, B);
             V3 sa0 = mk3(0.0f, 0.0f, 0.0f), sb0 = sa0;
             if (j < 4) {
@@ -1355,11 +1380,13 @@ __device__ void narrow_team(const Team &team, const Src &src, int ncand, const N
                 sb0 = mk3(q.b[3 * j], q.b[3 * j + 1], q.b[3 * j + 2]);
             }
             wave_sync();
+            const int nbase = claim();
+            const int seen = peek(nbase);
             // WIDE: a pair with a sphere goes to the 32-lane tier at once (a sphere's
             // support gives a new vertex at every iteration, so its polytopes grow)
             const bool sph = WIDE && (A.kind != SHAPE_BOX || B.kind != SHAPE_BOX);
             const EpaResult r = epa_group<16, ISL>(G, es, sa0, sb0, act && !sph, np);
-            if (act && !sph && j == 0 && r.status != EPA_SPILL) src.epa_result(c, r);
+            if (act && !sph && j == 0 && r.status != EPA_SPILL) src.epa_result(c, key, r);
             const uint64_t spill = ballot(act && j == 0 && (sph || r.status == EPA_SPILL));
             wave_sync();
             const int nsp = __popcll(spill);
@@ -1370,9 +1397,10 @@ __device__ void narrow_team(const Team &team, const Src &src, int ncand, const N
                 const bool act2 = k < nsp;
                 const int sg = nth_set_bit(spill, act2 ? k : 2 * t) >> 4;
                 const int cs = __shfl(c, 16 * sg, 64);
+                const int ks = __shfl(key, 16 * sg, 64);
                 const int sj = base + sg;
                 Shape SA, SB;
-                src.shapes(cs, SA, SB);
+                src.shapes(ks, SA, SB);
                 const EpaShapes e2 = epa_shapes_group(SA, SB);
                 V3 a2 = mk3(0.0f, 0.0f, 0.0f), b2 = a2;
                 if (j2 < 4) {
@@ -1382,16 +1410,17 @@ __device__ void narrow_team(const Team &team, const Src &src, int ncand, const N
                 }
                 wave_sync();
                 const EpaResult r2 = epa_group<32, ISL>(G2, e2, a2, b2, act2, np);
-                if (act2 && j2 == 0 && r2.status != EPA_SPILL) src.epa_result(cs, r2);
+                if (act2 && j2 == 0 && r2.status != EPA_SPILL) src.epa_result(cs, ks, r2);
                 uint64_t spill2 = ballot(act2 && j2 == 0 && r2.status == EPA_SPILL);
                 wave_sync();
                 while (spill2) {
                     const int src_lane = ffs64(spill2);  // 0 or 32
                     spill2 &= spill2 - 1;
                     const int cw = __shfl(cs, src_lane, 64);
+                    const int kw = __shfl(ks, src_lane, 64);
                     const int sw = __shfl(sj, src_lane, 64);
                     Shape WA, WB;
-                    src.shapes(cw, WA, WB);
+                    src.shapes(kw, WA, WB);
                     const EpaShapes ew = epa_shapes(WA, WB);
                     V3 p0 = mk3(0.0f, 0.0f, 0.0f);
                     if (lane < 4) {
@@ -1405,15 +1434,16 @@ __device__ void narrow_team(const Team &team, const Src &src, int ncand, const N
                     }
                     wave_sync();
                     const EpaResult rw = epa_wave(L, ew, p0, np);
-                    if (lane == 0) src.epa_result(cw, rw);
+                    if (lane == 0) src.epa_result(cw, kw, rw);
                     wave_sync();
                 }
             }
             for (uint64_t sp = WIDE ? 0ull : spill; sp; sp &= sp - 1) {
                 const int sg = ffs64(sp) >> 4;  // the spilled group
                 const int cs = __shfl(c, 16 * sg, 64);
+                const int ks = __shfl(key, 16 * sg, 64);
                 Shape SA, SB;
-                src.shapes(cs, SA, SB);
+                src.shapes(ks, SA, SB);
                 const EpaShapes ew = epa_shapes(SA, SB);
                 V3 p0 = mk3(0.0f, 0.0f, 0.0f);
                 if (lane < 4) {
@@ -1427,9 +1457,11 @@ __device__ void narrow_team(const Team &team, const Src &src, int ncand, const N
                 }
                 wave_sync();
                 const EpaResult rw = epa_wave(L, ew, p0, np);
-                if (lane == 0) src.epa_result(cs, rw);
+                if (lane == 0) src.epa_result(cs, ks, rw);
                 wave_sync();
             }
+            base = nbase;
+            c = resolve(base, seen);
         }
         team.sync();
         return;
@@ -1442,8 +1474,9 @@ __device__ void narrow_team(const Team &team, const Src &src, int ncand, const N
         job = __builtin_amdgcn_readfirstlane(job);
         if (job >= nepa) break;
         const int c = qidx[job];
+        const int key = qsim[job].key;
         Shape A, B;
-        src.shapes(c, A, B);
+        src.shapes(key, A, B);
         const EpaShapes es = epa_shapes(A, B);
         V3 p0 = mk3(0.0f, 0.0f, 0.0f);
         if (lane < 4) {
@@ -1457,7 +1490,7 @@ __device__ void narrow_team(const Team &team, const Src &src, int ncand, const N
         }
         wave_sync();
         const EpaResult r = epa_wave(L, es, p0, np);
-        if (lane == 0) src.epa_result(c, r);
+        if (lane == 0) src.epa_result(c, key, r);
         wave_sync();
     }
     team.sync();

@@ -145,29 +145,30 @@ struct WorldSrc {
     ContactRec *rec;
     int *flag;
     unsigned long long *cnt;
-    __device__ void shapes(int c, Shape &A, Shape &Bs) const
+    __device__ int key(int c) const { return cand[c]; }
+    __device__ void shapes(int key, Shape &A, Shape &Bs) const
     {
-        PairId q = SMALL ? decode_pair32(cand[c], nb, ns) : decode_pair(cand[c], nb, ns);
+        PairId q = SMALL ? decode_pair32(key, nb, ns) : decode_pair(key, nb, ns);
         int sa, sb;
         pair_slots(q, nb, sa, sb);
         A = body_shape(B, verts, sa, nb, *fl);
         Bs = body_shape(B, verts, sb, nb, *fl);
     }
     static constexpr bool kMemCorners = true;
-    __device__ void shapes_ref(int c, Shape &A, Shape &Bs) const
+    __device__ void shapes_ref(int key, Shape &A, Shape &Bs) const
     {
-        PairId q = SMALL ? decode_pair32(cand[c], nb, ns) : decode_pair(cand[c], nb, ns);
+        PairId q = SMALL ? decode_pair32(key, nb, ns) : decode_pair(key, nb, ns);
         int sa, sb;
         pair_slots(q, nb, sa, sb);
         A = body_shape_ref(B, verts, sa, nb, *fl);
         Bs = body_shape_ref(B, verts, sb, nb, *fl);
     }
     __device__ void gjk_result(int c, int g) const { flag[c] = 0; }
-    __device__ void epa_result(int c, const EpaResult &r) const
+    __device__ void epa_result(int c, int key, const EpaResult &r) const
     {
         if (r.status == EPA_OVERFLOW) atomicAdd(&cnt[CNT_EPA_OVERFLOW], 1ull);
         if (r.status != EPA_CONTACT) return;
-        PairId q = SMALL ? decode_pair32(cand[c], nb, ns) : decode_pair(cand[c], nb, ns);
+        PairId q = SMALL ? decode_pair32(key, nb, ns) : decode_pair(key, nb, ns);
         ContactRec &o = rec[c];
         int sa, sb;
         pair_slots(q, nb, sa, sb);
@@ -666,31 +667,57 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
         ci = __builtin_amdgcn_readfirstlane(s0.i);  // the same in every lane: kept scalar
         cj = __builtin_amdgcn_readfirstlane(s0.j);
     }
-    for (int64_t b = q0; b < q1; b += 64) {
+    // this lane's pair of the chunk starting at row ci, column cj; advances (ci, cj)
+    auto walk = [&](int &sa, int &sb) {
+        int acc = 0, row = ci, col = cj;
+        sa = sb = 0;
+        for (;;) {
+            const int len = nb - col;  // pairs (row, col..nb-1)
+            if (lane >= acc && lane < acc + len) {
+                sa = row;
+                sb = col + lane - acc;
+            }
+            if (acc + len >= 64 || row >= nb - 2) {  // the chunk ends in this row, or the triangle does
+                if (acc + len > 64) {
+                    ci = row;
+                    cj = col + 64 - acc;
+                } else {
+                    ci = row + 1;
+                    cj = row + 2;
+                }
+                break;
+            }
+            acc += len;
+            row += 1;
+            col = row + 1;
+        }
+    };
+    auto append = [&](bool k, int p) {
+        const uint64_t m = ballot(k);
+        const int r = mine + __popcll(m & ((1ull << lane) - 1ull));
+        if (k && r < S.cap) wbuf[r] = p;
+        mine += __popcll(m);
+    };
+    int64_t b = q0;
+    // two chunks at a time while both lie in the box-box triangle (their loads overlap)
+    for (; b + 128 <= ncc && b < q1; b += 128) {
+        int sa0, sb0, sa1, sb1;
+        walk(sa0, sb0);
+        walk(sa1, sb1);
+        const int p0 = (int)(b + lane), p1 = p0 + 64;
+        const V3 a0 = Lb.get3(F_PX, sa0), c0 = Lb.get3(F_PX, sb0), a1 = Lb.get3(F_PX, sa1), c1 = Lb.get3(F_PX, sb1);
+        const float ra0 = bradL[sa0], rc0 = bradL[sb0], ra1 = bradL[sa1], rc1 = bradL[sb1];
+        const bool k0 = p0 < q1 && cull_keep(a0, ra0, c0, rc0);
+        const bool k1 = p1 < q1 && cull_keep(a1, ra1, c1, rc1);
+        append(k0, p0);
+        append(k1, p1);
+    }
+    for (; b < q1; b += 64) {
         const int p = (int)(b + lane);
         bool k = false;
         if (b < ncc) {
-            int sa = 0, sb = 0, acc = 0, row = ci, col = cj;
-            for (;;) {
-                const int len = nb - col;  // pairs (row, col..nb-1)
-                if (lane >= acc && lane < acc + len) {
-                    sa = row;
-                    sb = col + lane - acc;
-                }
-                if (acc + len >= 64 || row >= nb - 2) {  // the chunk ends in this row, or the triangle does
-                    if (acc + len > 64) {
-                        ci = row;
-                        cj = col + 64 - acc;
-                    } else {
-                        ci = row + 1;
-                        cj = row + 2;
-                    }
-                    break;
-                }
-                acc += len;
-                row += 1;
-                col = row + 1;
-            }
+            int sa, sb;
+            walk(sa, sb);
             if (p < q1 && p < ncc) k = cull_keep(Lb.get3(F_PX, sa), bradL[sa], Lb.get3(F_PX, sb), bradL[sb]);
         }
         if (p >= ncc && p < q1) {
@@ -703,10 +730,7 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
             slot_sphere(Lb, bradL, sb, fl, cb, rb);
             k = cull_keep(ca, ra, cb, rb);
         }
-        const uint64_t m = ballot(k);
-        const int r = mine + __popcll(m & ((1ull << lane) - 1ull));
-        if (k && r < S.cap) wbuf[r] = p;
-        mine += __popcll(m);
+        append(k, p);
     }
     int ncand = mine;
     if (NW > 1) {
