@@ -889,6 +889,14 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
         if (j < 4) F = make_face(Pa, Pb, Pc, a, b, c, ca, cb, cc);
     }
     int nf = 4, nv = 4, iter = 0;
+    // A group that converges keeps its closest face (and, for the islands choice,
+    // face 0's distance and the winner's index) and stops: its polytope is frozen
+    // from then on, so the contact points are computed once for every group
+    // after the loop rather than on each iteration where some group converges.
+    bool cv = false;
+    V3 cN = z;
+    float ccur = 0.0f, cd0 = 0.0f;
+    int cfid = 0, cbest = 0;
     for (;;) {
         // while (iter++ <= MAX_EPA_ITERATIONS), per group
         if (run && !(iter <= p.max_epa_iterations)) run = false;  // status stays EPA_ITER
@@ -913,59 +921,14 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
         // new support along the stored normal
         const Vtx ns = group_support<G>(sh, N);
         const bool conv = run && dot(N, ns.P) - cur < p.epa_tolerance;
-        if (ballot(conv)) {
-            V3 Nc = N;
-            float cc = cur;
-            double depth = (double)cur;
-            if constexpr (ISL) {
-                // the final face by binary64 plane distance: the first of the least
-                // below face 0's, else face 0 (epa_wave's islands choice per group)
-                wave_sync();
-                const double d64 = j < nf ? group_face_dist64(L.vp[g], F) : 0.0;
-                const double f0 = __shfl(d64, gb, 64);
-                bool v64 = conv && j < nf && d64 < f0;
-                double bd = v64 ? d64 : 0.0;
-                int bi = v64 ? j : 0x7fffffff;
-                group_argmin64<G>(v64, bd, bi);
-                const int sel = v64 ? bi : 0;
-                depth = v64 ? bd : f0;
-                const int so = gb + sel;
-                const V3 Ns = mk3(gpull_f(F.nx, so), gpull_f(F.ny, so), gpull_f(F.nz, so));
-                const float ds = gpull_f(F.d, so);
-                const int fs = gpull_i(F.id, so);
-                if (sel != best) {
-                    Nc = Ns;
-                    cc = sel == 0 ? d0 : ds;
-                    fid = fs;
-                }
-            }
-            const int a = fid & 255, b = (fid >> 8) & 255, c = (fid >> 16) & 255;
-            const int am = gb + (a & (G - 1)), bm = gb + (b & (G - 1)), cm = gb + (c & (G - 1));
-            const V3 PA = gpull3(VP, am), PB = gpull3(VP, bm), PC = gpull3(VP, cm);
-            const V3 AA = gpull3(VA, am), AB = gpull3(VA, bm), AC = gpull3(VA, cm);
-            const V3 BA = gpull3(VB, am), BB = gpull3(VB, bm), BC = gpull3(VB, cm);
-            if (conv) {
-                const V3 P = mul(Nc, cc);
-                const V3 e0 = sub(PB, PA), e1 = sub(PC, PA), e2 = sub(P, PA);
-                const float d00 = dot(e0, e0), d01 = dot(e0, e1), d11 = dot(e1, e1);
-                const float d20 = dot(e2, e0), d21 = dot(e2, e1);
-                const float denom = d00 * d11 - d01 * d01;
-                const float V = (d11 * d20 - d01 * d21) / denom;
-                const float W = (d00 * d21 - d01 * d20) / denom;
-                const float U = 1.0f - V - W;
-                if (fabsf(U) > 1.0f || fabsf(V) > 1.0f || fabsf(W) > 1.0f) {
-                    r.status = EPA_BARY;
-                } else if (!isfinite(U) || !isfinite(V) || !isfinite(W)) {
-                    r.status = EPA_NONFINITE;
-                } else {
-                    r.pa = add(add(mul(AA, U), mul(AB, V)), mul(AC, W));
-                    r.n = mul(Nc, -1.0f);
-                    r.pb = add(add(mul(BA, U), mul(BB, V)), mul(BC, W));
-                    r.depth = depth;
-                    r.status = EPA_CONTACT;
-                }
-                run = false;
-            }
+        if (conv) {
+            cv = true;
+            cN = N;
+            ccur = cur;
+            cd0 = d0;
+            cfid = fid;
+            cbest = best;
+            run = false;
         }
         if (run && nv >= G) {
             r.status = EPA_SPILL;
@@ -1104,6 +1067,60 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
             if (nf > r.max_faces) r.max_faces = nf;
         }
         wave_sync();
+    }
+    if (ballot(cv)) {
+        V3 Nc = cN;
+        float cc = ccur;
+        double depth = (double)ccur;
+        int fid = cfid;
+        if constexpr (ISL) {
+            // the final face by binary64 plane distance: the first of the least
+            // below face 0's, else face 0 (epa_wave's islands choice per group)
+            wave_sync();
+            const double d64 = j < nf ? group_face_dist64(L.vp[g], F) : 0.0;
+            const double f0 = __shfl(d64, gb, 64);
+            bool v64 = cv && j < nf && d64 < f0;
+            double bd = v64 ? d64 : 0.0;
+            int bi = v64 ? j : 0x7fffffff;
+            group_argmin64<G>(v64, bd, bi);
+            const int sel = v64 ? bi : 0;
+            depth = v64 ? bd : f0;
+            const int so = gb + sel;
+            const V3 Ns = mk3(gpull_f(F.nx, so), gpull_f(F.ny, so), gpull_f(F.nz, so));
+            const float ds = gpull_f(F.d, so);
+            const int fs = gpull_i(F.id, so);
+            if (sel != cbest) {
+                Nc = Ns;
+                cc = sel == 0 ? cd0 : ds;
+                fid = fs;
+            }
+        }
+        const int a = fid & 255, b = (fid >> 8) & 255, c = (fid >> 16) & 255;
+        const int am = gb + (a & (G - 1)), bm = gb + (b & (G - 1)), cm = gb + (c & (G - 1));
+        const V3 PA = gpull3(VP, am), PB = gpull3(VP, bm), PC = gpull3(VP, cm);
+        const V3 AA = gpull3(VA, am), AB = gpull3(VA, bm), AC = gpull3(VA, cm);
+        const V3 BA = gpull3(VB, am), BB = gpull3(VB, bm), BC = gpull3(VB, cm);
+        if (cv) {
+            const V3 P = mul(Nc, cc);
+            const V3 e0 = sub(PB, PA), e1 = sub(PC, PA), e2 = sub(P, PA);
+            const float d00 = dot(e0, e0), d01 = dot(e0, e1), d11 = dot(e1, e1);
+            const float d20 = dot(e2, e0), d21 = dot(e2, e1);
+            const float denom = d00 * d11 - d01 * d01;
+            const float V = (d11 * d20 - d01 * d21) / denom;
+            const float W = (d00 * d21 - d01 * d20) / denom;
+            const float U = 1.0f - V - W;
+            if (fabsf(U) > 1.0f || fabsf(V) > 1.0f || fabsf(W) > 1.0f) {
+                r.status = EPA_BARY;
+            } else if (!isfinite(U) || !isfinite(V) || !isfinite(W)) {
+                r.status = EPA_NONFINITE;
+            } else {
+                r.pa = add(add(mul(AA, U), mul(AB, V)), mul(AC, W));
+                r.n = mul(Nc, -1.0f);
+                r.pb = add(add(mul(BA, U), mul(BB, V)), mul(BC, W));
+                r.depth = depth;
+                r.status = EPA_CONTACT;
+            }
+        }
     }
     return r;
 }
