@@ -699,8 +699,10 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
         mine += __popcll(m);
     };
     int64_t b = q0;
-    // two chunks at a time while both lie in the box-box triangle (their loads overlap)
-    for (; b + 128 <= ncc && b < q1; b += 128) {
+    // two chunks at a time while both lie in the box-box triangle (their loads
+    // overlap); not in the warm-started instantiation, where the extra registers
+    // make the compiler spill
+    for (; !WARM && b + 128 <= ncc && b < q1; b += 128) {
         int sa0, sb0, sa1, sb1;
         walk(sa0, sb0);
         walk(sa1, sb1);
