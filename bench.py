@@ -533,7 +533,8 @@ def main():
         qctx.set_state(sc)  # empties the warm-start lists
         qctx.step_batched(args.mode_steps)
         modes["batched_warm"] = qctx.batched_timing()["elapsed_ms"] / args.mode_steps
-        # the FP64-islands mode on the same batched steps
+        # the FP64-islands mode on the same batched steps (cold, as the euler figure)
+        qctx.set_warm_start(False)
         qctx.set_params(dataclasses.replace(sc.params, fp64_islands=1))
         qctx.set_state(sc)
         qctx.step_batched(2)

@@ -897,6 +897,14 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
     V3 cN = z;
     float ccur = 0.0f, cd0 = 0.0f;
     int cfid = 0, cbest = 0;
+    // the horizon's edge maps (below), clear at the start of every trip
+    constexpr int MW = G * G / 32;
+    static_assert(MW <= G, "a group's lanes clear its edge maps");
+    if (j < MW) {
+        L.m1[g][j] = 0u;
+        L.m2[g][j] = 0u;
+    }
+    wave_sync();
     for (;;) {
         // while (iter++ <= MAX_EPA_ITERATIONS), per group
         if (run && !(iter <= p.max_epa_iterations)) run = false;  // status stays EPA_ITER
@@ -958,14 +966,9 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
         // group's edge maps; bits of this lane's three edges: same key seen twice
         // (s2), reverse key seen (r1) and seen twice (r2).  Edges with a NaN class
         // never match, as vertex == never holds for them.
+        // The maps are clear here: cleared before the loop and after each trip's
+        // last read of them.
         const int cl = F.cl;
-        constexpr int MW = G * G / 32;
-        static_assert(MW <= G, "a group's lanes clear its edge maps");
-        if (j < MW) {
-            L.m1[g][j] = 0u;
-            L.m2[g][j] = 0u;
-        }
-        wave_sync();
         if (w0) {
 #pragma unroll
             for (int e = 0; e < 3; ++e) {
@@ -1022,6 +1025,10 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
             }
         }
         wave_sync();
+        if (j < MW) {  // every read of the maps is behind the sync above
+            L.m1[g][j] = 0u;
+            L.m2[g][j] = 0u;
+        }
         int ea = 0, eb = 0, eca = 0, ecb = 0;
         if (run && j < ne) {
             const int k2 = L.ekey[gb + j], i2 = L.eids[gb + j];

@@ -698,23 +698,7 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
         if (k && r < S.cap) wbuf[r] = p;
         mine += __popcll(m);
     };
-    int64_t b = q0;
-    // two chunks at a time while both lie in the box-box triangle (their loads
-    // overlap); not in the warm-started instantiation, where the extra registers
-    // make the compiler spill
-    for (; !WARM && b + 128 <= ncc && b < q1; b += 128) {
-        int sa0, sb0, sa1, sb1;
-        walk(sa0, sb0);
-        walk(sa1, sb1);
-        const int p0 = (int)(b + lane), p1 = p0 + 64;
-        const V3 a0 = Lb.get3(F_PX, sa0), c0 = Lb.get3(F_PX, sb0), a1 = Lb.get3(F_PX, sa1), c1 = Lb.get3(F_PX, sb1);
-        const float ra0 = bradL[sa0], rc0 = bradL[sb0], ra1 = bradL[sa1], rc1 = bradL[sb1];
-        const bool k0 = p0 < q1 && cull_keep(a0, ra0, c0, rc0);
-        const bool k1 = p1 < q1 && cull_keep(a1, ra1, c1, rc1);
-        append(k0, p0);
-        append(k1, p1);
-    }
-    for (; b < q1; b += 64) {
+    for (int64_t b = q0; b < q1; b += 64) {
         const int p = (int)(b + lane);
         bool k = false;
         if (b < ncc) {
