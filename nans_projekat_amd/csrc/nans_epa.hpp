@@ -111,7 +111,9 @@ __device__ __forceinline__ void st3(float (*a)[3], int i, V3 v)
 }
 
 // DPP move of a 32-bit value (GFX9 encodings: quad_perm 0x00-0xff, row_mirror
-// 0x140, row_half_mirror 0x141); every source lane of these patterns exists.
+// 0x140, row_half_mirror 0x141; gfx90a and later: row_newbcast 0x150-0x15f,
+// lane n of each 16-lane row to the whole row); every source lane of these
+// patterns exists.
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v)
 {
@@ -717,6 +719,26 @@ __device__ __forceinline__ V3 gpull3(V3 v, int src_lane)
 {
     return mk3(gpull_f(v.x, src_lane), gpull_f(v.y, src_lane), gpull_f(v.z, src_lane));
 }
+// lane U of this lane's G-lane group, in every lane (G = 16: one row broadcast,
+// no LDS round trip); the whole wave must be active
+template <int G, int U>
+__device__ __forceinline__ float gbcast_f(float v)
+{
+    if constexpr (G == 16) return dpp_f<0x150 + U>(v);
+    else return gpull_f(v, (lane_id() & (64 - G)) + U);
+}
+template <int G, int U>
+__device__ __forceinline__ int gbcast_i(int v)
+{
+    if constexpr (G == 16) return dpp_i<0x150 + U>(v);
+    else return gpull_i(v, (lane_id() & (64 - G)) + U);
+}
+template <int G, int U>
+__device__ __forceinline__ V3 gbcast3(V3 v)
+{
+    return mk3(gbcast_f<G, U>(v.x), gbcast_f<G, U>(v.y), gbcast_f<G, U>(v.z));
+}
+
 // this lane's G-lane group bits of a wave ballot
 template <int G>
 __device__ __forceinline__ uint32_t gbits(uint64_t m)
@@ -864,14 +886,16 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
     V3 VA = j < 4 ? sa0 : z, VB = j < 4 ? sb0 : z;
     V3 VP = j < 4 ? sub(sa0, sb0) : z;
     if (j < 4) st3(L.vp[g], j, VP);
+    // the group's four simplex vertices in every lane of the group
+    const V3 P0 = gbcast3<G, 0>(VP), P1 = gbcast3<G, 1>(VP), P2 = gbcast3<G, 2>(VP), P3 = gbcast3<G, 3>(VP);
     int c0;
     {
         int cls = j;
         bool found = false;
+        const V3 pu[3] = {P0, P1, P2};
 #pragma unroll
         for (int u = 0; u < 3; ++u) {
-            const V3 pu = gpull3(VP, gb + u);
-            if (!found && u < j && veq(pu, VP)) {
+            if (!found && u < j && veq(pu[u], VP)) {
                 cls = u;
                 found = true;
             }
@@ -884,8 +908,16 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
         const int a = j == 3 ? 1 : 0;
         const int b = j == 0 ? 1 : (j == 1 ? 2 : 3);
         const int c = j == 0 ? 2 : (j == 1 ? 3 : (j == 2 ? 1 : 2));
-        const V3 Pa = gpull3(VP, gb + a), Pb = gpull3(VP, gb + b), Pc = gpull3(VP, gb + c);
-        const int ca = gpull_i(c0, gb + a), cb = gpull_i(c0, gb + b), cc = gpull_i(c0, gb + c);
+        const int C0 = gbcast_i<G, 0>(c0), C1 = gbcast_i<G, 1>(c0), C2 = gbcast_i<G, 2>(c0), C3 = gbcast_i<G, 3>(c0);
+        V3 Pa = P0, Pb = P1, Pc = P1;
+        fsel(Pa, a == 1, P1);
+        fsel(Pb, b == 2, P2);
+        fsel(Pb, b == 3, P3);
+        fsel(Pc, c == 2, P2);
+        fsel(Pc, c == 3, P3);
+        const int ca = a == 1 ? C1 : C0;
+        const int cb = b == 1 ? C1 : (b == 2 ? C2 : C3);
+        const int cc = c == 1 ? C1 : (c == 2 ? C2 : C3);
         if (j < 4) F = make_face(Pa, Pb, Pc, a, b, c, ca, cb, cc);
     }
     int nf = 4, nv = 4, iter = 0;
@@ -916,7 +948,7 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
         }
         if (!ballot(run)) break;
         // closest face: the first face holding the minimum below face 0's distance
-        const float d0 = gpull_f(gabs(dot(mk3(F.nx, F.ny, F.nz), mk3(F.ax, F.ay, F.az))), gb);
+        const float d0 = gbcast_f<G, 0>(gabs(dot(mk3(F.nx, F.ny, F.nz), mk3(F.ax, F.ay, F.az))));
         const float k0 = (j < nf && F.d < d0) ? F.d : __builtin_inff();
         const float m = group_min<G>(k0);
         const uint32_t b0 = gbits(ballot(k0 == m));
