@@ -174,7 +174,13 @@ __device__ __forceinline__ V3 gjk_first_dir(const Shape &a, const Shape &b) { re
 // per pair by the caller.
 __device__ __forceinline__ int evolve_simplex(const Shape &a, const Shape &b, Simplex &S, int fp64, V3 d0)
 {
+    // The cases only choose the direction and which vertices leave; the simplex is
+    // rearranged once after them (erase er1, then er2; swap s1 and s2), so that
+    // lanes in different cases share one copy of the register moves.  Every case
+    // reads the simplex before it changes, as the reference's does.
     V3 dir = d0;
+    int er1 = -1, er2 = -1;
+    bool swap12 = false;
     if (S.n == 1) {
         dir = mul(dir, -1.0f);
     } else if (S.n == 2) {
@@ -188,11 +194,8 @@ __device__ __forceinline__ int evolve_simplex(const Shape &a, const Shape &b, Si
         if (U <= 0.0f) cp = B;
         else if (V <= 0.0f) cp = A;
         else cp = add(mul(A, U), mul(B, V));
-        if (V <= 0.0f) {
-            simplex_erase(S, 1);
-        } else if (U <= 0.0f) {
-            simplex_erase(S, 0);
-        }
+        if (V <= 0.0f) er1 = 1;
+        else if (U <= 0.0f) er1 = 0;
         dir = neg(cp);
     } else if (S.n == 3) {
         V3 n2o = neg(S.s0.P);
@@ -204,31 +207,29 @@ __device__ __forceinline__ int evolve_simplex(const Shape &a, const Shape &b, Si
         if (dot(e2n, n2o) > 0.0f) {
             if (dot(e2, n2o) > 0.0f) {
                 dir = triple_cross(e2, n2o, e2);
-                simplex_erase(S, 1);
+                er1 = 1;
             } else if (dot(e1, n2o) > 0.0f) {
                 dir = triple_cross(e1, n2o, e1);
-                simplex_erase(S, 2);
+                er1 = 2;
             } else {
                 dir = n2o;
-                simplex_erase(S, 2);
-                simplex_erase(S, 1);
+                er1 = 2;
+                er2 = 1;
             }
         } else if (dot(e1n, n2o) > 0.0f) {
             if (dot(e1, n2o) > 0.0f) {
                 dir = triple_cross(e1, n2o, e1);
-                simplex_erase(S, 2);
+                er1 = 2;
             } else {
                 dir = n2o;
-                simplex_erase(S, 2);
-                simplex_erase(S, 1);
+                er1 = 2;
+                er2 = 1;
             }
         } else if (dot(tn, n2o) > 0.0f) {
             dir = tn;
         } else {
             dir = neg(tn);
-            Vtx t = S.s1;
-            S.s1 = S.s2;
-            S.s2 = t;
+            swap12 = true;
         }
     } else if (S.n == 4) {
         V3 DA = sub(S.s0.P, S.s3.P);
@@ -239,18 +240,25 @@ __device__ __forceinline__ int evolve_simplex(const Shape &a, const Shape &b, Si
         V3 bcd = cross(DB, DC);
         V3 cad = cross(DC, DA);
         if (dot(abd, D0) > 0.0f) {
-            simplex_erase(S, 2);
+            er1 = 2;
             dir = abd;
         } else if (dot(bcd, D0) > 0.0f) {
-            simplex_erase(S, 0);
+            er1 = 0;
             dir = bcd;
         } else if (dot(cad, D0) > 0.0f) {
-            simplex_erase(S, 1);
+            er1 = 1;
             dir = cad;
         } else {
             return GJK_FOUND;
         }
     }
+    if (swap12) {
+        const Vtx t = S.s1;
+        vsel(S.s1, true, S.s2);
+        vsel(S.s2, true, t);
+    }
+    if (er1 >= 0) simplex_erase(S, er1);
+    if (er2 >= 0) simplex_erase(S, er2);
     if (fp64) {
         double dx = (double)dir.x, dy = (double)dir.y, dz = (double)dir.z;
         if (__builtin_sqrt((dx * dx + dy * dy) + dz * dz) <= (double)0.0001f) return GJK_NO;
