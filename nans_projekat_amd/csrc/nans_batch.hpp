@@ -141,6 +141,27 @@ struct nans_worlds {
     int *blk_offsets = nullptr;
     int *ncand = nullptr;
     int max_blk = 0;
+    // Further lanes of the plain step for batches of several worlds: world w runs on
+    // lane w % lanes (lane 0 is the context's stream with the scratch above; a world
+    // that takes the grid broadphase always runs on lane 0, the grid's only user).
+    struct PlainLane {
+        hipStream_t s = nullptr;
+        hipEvent_t ev = nullptr;
+        int *cand = nullptr;
+        nans::ContactRec *rec = nullptr;
+        int *flag = nullptr;
+        nans::ContactRec *con = nullptr;
+        int *done = nullptr;
+        int *qidx = nullptr;
+        nans::SimplexRec *qsim = nullptr;
+        int *blk_counts = nullptr;
+        int *blk_offsets = nullptr;
+        int *ncand = nullptr;
+        int *warm_cp = nullptr;
+        float *warm_sums = nullptr;
+    };
+    std::vector<PlainLane> plain_lanes;  // lanes 1.. (lane 0 is not stored)
+    hipEvent_t plain_ev0 = nullptr;
     // hierarchical-grid broadphase of the plain step (sized for the largest world)
     int bp_buckets = 0;          // power of two >= 2 * nmax
     long long *bp_cell = nullptr;  // [3][nmax] cell coordinates

@@ -943,6 +943,21 @@ static void release_recording(nans_worlds *W)
     W->rec_cap = nullptr;
 }
 
+static void release_plain_lanes(nans_worlds *W)
+{
+    for (auto &L : W->plain_lanes) {
+        for (void *q : {(void *)L.cand, (void *)L.rec, (void *)L.flag, (void *)L.con, (void *)L.done,
+                        (void *)L.qidx, (void *)L.qsim, (void *)L.blk_counts, (void *)L.blk_offsets,
+                        (void *)L.ncand, (void *)L.warm_cp, (void *)L.warm_sums})
+            wfree(q);
+        if (L.ev) hipEventDestroy(L.ev);
+        if (L.s) hipStreamDestroy(L.s);
+    }
+    W->plain_lanes.clear();
+    if (W->plain_ev0) hipEventDestroy(W->plain_ev0);
+    W->plain_ev0 = nullptr;
+}
+
 int worlds_release(nans_ctx *ctx)
 {
     if (!ctx || !ctx->worlds) return NANS_OK;
@@ -963,6 +978,7 @@ int worlds_release(nans_ctx *ctx)
         if (ev) hipEventDestroy(ev);
     release_batch_scratch(W);
     release_recording(W);
+    release_plain_lanes(W);
     adjoint_release(W);
     delete W;
     ctx->worlds = nullptr;
@@ -1473,6 +1489,41 @@ int nans_reset_stats(nans_ctx *ctx)
 }
 
 
+// Lanes of the plain step: as many as worlds, at most PLAIN_LANES (each a stream
+// with its own scratch).  The worlds of a batch are independent, and one world's
+// launches are short, so several in flight keep the GPU busy.
+constexpr int PLAIN_LANES = 8;
+
+static int plain_lanes_prepare(nans_ctx *ctx, int lanes)
+{
+    nans_worlds *W = ctx->worlds;
+    const size_t cap = (size_t)W->cand_cap;
+    if (!W->plain_ev0) NANS_HIP(hipEventCreateWithFlags(&W->plain_ev0, hipEventDisableTiming));
+    while ((int)W->plain_lanes.size() < lanes - 1) {
+        W->plain_lanes.emplace_back();
+        nans_worlds::PlainLane &L = W->plain_lanes.back();
+        NANS_HIP(hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking));
+        NANS_HIP(hipEventCreateWithFlags(&L.ev, hipEventDisableTiming));
+        NANS_TRY(dmalloc(&L.cand, cap));
+        NANS_TRY(dmalloc(&L.rec, cap));
+        NANS_TRY(dmalloc(&L.flag, cap));
+        NANS_TRY(dmalloc(&L.con, cap));
+        NANS_TRY(dmalloc(&L.done, cap));
+        NANS_TRY(dmalloc(&L.qidx, cap));
+        NANS_TRY(dmalloc(&L.qsim, cap));
+        NANS_TRY(dmalloc(&L.blk_counts, (size_t)W->max_blk));
+        NANS_TRY(dmalloc(&L.blk_offsets, (size_t)W->max_blk));
+        NANS_TRY(dmalloc(&L.ncand, 1));
+    }
+    if (ctx->warm) {
+        for (auto &L : W->plain_lanes) {
+            if (!L.warm_cp) NANS_TRY(dmalloc(&L.warm_cp, cap));
+            if (!L.warm_sums) NANS_TRY(dmalloc(&L.warm_sums, 3 * cap));
+        }
+    }
+    return NANS_OK;
+}
+
 int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
 {
     NANS_REQUIRE(ctx && ctx->worlds, NANS_ERR_STATE, "no worlds uploaded");
@@ -1484,40 +1535,55 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
     NANS_TRY(setup_floor(ctx, s));
     NANS_TRY(snapshot_take(ctx, s));
     const StepConsts k = step_consts(ctx->p, ctx->orientation);
+    const int lanes = std::min(PLAIN_LANES, W->nworlds);
+    NANS_TRY(plain_lanes_prepare(ctx, lanes));
+    // the lanes start after the set-up above on the context's stream
+    NANS_HIP(hipEventRecord(W->plain_ev0, s));
+    for (int l = 1; l < lanes; ++l) NANS_HIP(hipStreamWaitEvent(W->plain_lanes[l - 1].s, W->plain_ev0, 0));
     for (int step = 0; step < nsteps; ++step) {
         for (int w = 0; w < W->nworlds; ++w) {
             const int b0 = W->h_off[w], n = W->h_off[w + 1] - b0, nb = W->h_nb[w], ns = n - nb;
             if (n == 0) continue;
+            // this world's lane: its stream and scratch (always the same lane for a
+            // world, so its steps stay in order)
+            const int l = use_grid(ctx, n) ? 0 : w % lanes;
+            const nans_worlds::PlainLane *L = l ? &W->plain_lanes[l - 1] : nullptr;
+            hipStream_t ls = L ? L->s : s;
+            int *cand = L ? L->cand : W->cand, *flag = L ? L->flag : W->flag, *done = L ? L->done : W->done;
+            int *qidx = L ? L->qidx : W->qidx, *ncand = L ? L->ncand : W->ncand;
+            int *blk_counts = L ? L->blk_counts : W->blk_counts, *blk_offsets = L ? L->blk_offsets : W->blk_offsets;
+            ContactRec *rec = L ? L->rec : W->rec, *con = L ? L->con : W->con;
+            SimplexRec *qsim = L ? L->qsim : W->qsim;
             Planes B{W->planes + b0, (int)W->N};
             float *verts = W->verts + (int64_t)24 * b0;
             float *brad = W->brad + b0;
             const int64_t npairs = pair_count(nb, ns);
             const int nblk = (int)((npairs + CULL_CHUNK - 1) / CULL_CHUNK);
-            k_plain_forces<<<(n + 255) / 256, 256, 0, s>>>(B, verts, brad, n, nb, k);
+            k_plain_forces<<<(n + 255) / 256, 256, 0, ls>>>(B, verts, brad, n, nb, k);
             if (nblk > 0) {
                 if (use_grid(ctx, n)) {
                     grid_broadphase(W, B, brad, nb, ns, npairs, s);
                 } else {
-                    k_plain_cull<false><<<nblk, 256, 0, s>>>(B, brad, nb, ns, npairs, W->floor, W->blk_counts,
-                                                             nullptr, nullptr, 0, W->cnt);
-                    scan_counts(W->blk_counts, nblk, W->blk_offsets, W->ncand, W->cand_cap, W->cnt, s);
-                    k_plain_cull<true><<<nblk, 256, 0, s>>>(B, brad, nb, ns, npairs, W->floor, nullptr,
-                                                            W->blk_offsets, W->cand, W->cand_cap, W->cnt);
+                    k_plain_cull<false><<<nblk, 256, 0, ls>>>(B, brad, nb, ns, npairs, W->floor, blk_counts,
+                                                              nullptr, nullptr, 0, W->cnt);
+                    scan_counts(blk_counts, nblk, blk_offsets, ncand, W->cand_cap, W->cnt, ls);
+                    k_plain_cull<true><<<nblk, 256, 0, ls>>>(B, brad, nb, ns, npairs, W->floor, nullptr,
+                                                             blk_offsets, cand, W->cand_cap, W->cnt);
                 }
                 int grid = std::max(1, std::min((W->cand_cap + NARROW_RANGE - 1) / NARROW_RANGE, 2048));
-                k_plain_narrow<<<grid, 256, 0, s>>>(B, verts, nb, ns, W->floor, W->cand, W->ncand, k.np, W->rec,
-                                                    W->flag, W->qidx, W->qsim, W->cnt);
+                k_plain_narrow<<<grid, 256, 0, ls>>>(B, verts, nb, ns, W->floor, cand, ncand, k.np, rec, flag,
+                                                     qidx, qsim, W->cnt);
             } else {
-                k_zero_i32<<<1, 64, 0, s>>>(W->ncand, 1);
+                k_zero_i32<<<1, 64, 0, ls>>>(ncand, 1);
             }
             SolveArgs a;
             a.B = B;
             a.n = n;
-            a.flag = W->flag;
-            a.rec = W->rec;
-            a.ncand = W->ncand;
-            a.con = W->con;
-            a.done = W->done;
+            a.flag = flag;
+            a.rec = rec;
+            a.ncand = ncand;
+            a.con = con;
+            a.done = done;
             a.con_cap = W->cand_cap;
             a.k = k;
             a.fl = W->floor;
@@ -1527,7 +1593,7 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
             a.rec_cap = W->record ? W->h_rec_cap[w] : 0;
             a.rec_count = W->record ? W->rec_count + w : nullptr;
             a.cnt = W->cnt;
-            a.cand = W->cand;
+            a.cand = cand;
             if (ctx->warm) {
                 const int cur = W->warm_flip, nxt = cur ^ 1;
                 const int64_t off = W->h_warm_off[w];
@@ -1538,15 +1604,20 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
                 a.warm_s_next = W->warm_s[nxt] + 3 * off;
                 a.warm_n_next = W->warm_n[nxt] + w;
                 a.warm_cap = W->h_warm_cap[w];
-                a.warm_cp = W->warm_cp;
-                a.warm_sums = W->warm_sums;
-                k_plain_solve<true><<<1, SOLVE_THREADS, sizeof(int) * n, s>>>(a);
+                a.warm_cp = L ? L->warm_cp : W->warm_cp;
+                a.warm_sums = L ? L->warm_sums : W->warm_sums;
+                k_plain_solve<true><<<1, SOLVE_THREADS, sizeof(int) * n, ls>>>(a);
             } else {
-                k_plain_solve<false><<<1, SOLVE_THREADS, sizeof(int) * n, s>>>(a);
+                k_plain_solve<false><<<1, SOLVE_THREADS, sizeof(int) * n, ls>>>(a);
             }
             NANS_HIP(hipGetLastError());
         }
         if (ctx->warm) W->warm_flip ^= 1;
+    }
+    // the context's stream waits for every lane
+    for (int l = 1; l < lanes; ++l) {
+        NANS_HIP(hipEventRecord(W->plain_lanes[l - 1].ev, W->plain_lanes[l - 1].s));
+        NANS_HIP(hipStreamWaitEvent(s, W->plain_lanes[l - 1].ev, 0));
     }
     NANS_HIP(hipStreamSynchronize(s));
     return step_finish(ctx);
