@@ -15,6 +15,11 @@ constexpr int PLAIN_MAX_BODIES = 16384;  // solver head[] lives in LDS (64 KiB)
 constexpr int BATCH_MAX_BODIES = 512;    // bodies of one world kept in LDS by the batched path
 constexpr int SCAN_THREADS = 1024;
 constexpr int SOLVE_THREADS = 1024;
+// The plain step's solve for worlds of at most PLAIN_SMALL_BODIES bodies: 256
+// threads, and the terms of the first PLAIN_SMALL_TERMS contacts kept in LDS.
+constexpr int PLAIN_SMALL_THREADS = 256;
+constexpr int PLAIN_SMALL_BODIES = 512;
+constexpr int PLAIN_SMALL_TERMS = 320;
 constexpr int CULL_CHUNK = 4096;          // canonical pairs per block of the plain cull
 
 enum : int {

@@ -486,6 +486,7 @@ struct SolveArgs {
     StepConsts k;
     const FloorDev *fl;
     float Cf, Cr;
+    int npre;             // contact terms kept in the dynamic LDS after head[]
     ContactRec *rec_out;  // recording (may be null)
     int rec_cap;
     int *rec_count;
@@ -505,22 +506,25 @@ struct SolveArgs {
 };
 
 // Ordered contact compaction, dependency-round solve and IntegrateVelocities of
-// one world, by one workgroup; head[] in dynamic LDS.
-template <bool WARM>
-__global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
+// one world, by one workgroup of T threads; head[] in dynamic LDS, followed by
+// npre contact terms (T = PLAIN_SMALL_THREADS for small worlds, whose 256 threads
+// have the registers to keep the terms' application free of spills; the
+// 1024-thread variant for large worlds keeps none).
+template <bool WARM, int T = SOLVE_THREADS>
+__global__ __launch_bounds__(T) void k_plain_solve(SolveArgs a)
 {
     extern __shared__ int head[];
-    __shared__ int wave_tot[SOLVE_THREADS / 64];
+    __shared__ int wave_tot[T / 64];
     __shared__ FloorDev fl;
     if (threadIdx.x == 0) fl = *a.fl;
     __syncthreads();
     const int total = *a.ncand;
     int ncon = 0;
-    for (int base = 0; base < total; base += SOLVE_THREADS) {
+    for (int base = 0; base < total; base += T) {
         int c = base + threadIdx.x;
         bool hit = c < total && a.flag[c];
         int tot;
-        int r = block_rank<SOLVE_THREADS>(hit, wave_tot, tot);
+        int r = block_rank<T>(hit, wave_tot, tot);
         if (hit && ncon + r < a.con_cap) {
             a.con[ncon + r] = a.rec[c];
             if (WARM) a.warm_cp[ncon + r] = a.cand[c];
@@ -536,7 +540,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
     if (WARM) {
         // accumulators of the same pair in the previous step's list, else zero
         const int np = *a.warm_n_prev;
-        for (int c = threadIdx.x; c < ncon; c += SOLVE_THREADS) {
+        for (int c = threadIdx.x; c < ncon; c += T) {
             const int key = a.warm_cp[c];
             int lo = 0, hi = np;
             while (lo < hi) {
@@ -549,24 +553,31 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_plain_solve(SolveArgs a)
         }
         __syncthreads();
     }
-    solve_rounds<SOLVE_THREADS, WARM>(a.B, a.n, DirectCon{a.con}, ncon, a.done, head, fl, a.k, a.Cf, a.Cr,
-                                      PreTerms{}, a.warm_sums);
+    if constexpr (T == SOLVE_THREADS) {  // no terms kept: the path that applies them is not compiled
+        solve_rounds<T, WARM>(a.B, a.n, DirectCon{a.con}, ncon, a.done, head, fl, a.k, a.Cf, a.Cr, PreTerms{},
+                              a.warm_sums);
+    } else {
+        PreTerms pre;
+        pre.a = reinterpret_cast<ContactTerms *>(head + ((a.n + 3) & ~3));
+        pre.na = a.npre;
+        solve_rounds<T, WARM>(a.B, a.n, DirectCon{a.con}, ncon, a.done, head, fl, a.k, a.Cf, a.Cr, pre, a.warm_sums);
+    }
     if (WARM) {
         __syncthreads();
         const int m = min(ncon, a.warm_cap);
-        for (int c = threadIdx.x; c < m; c += SOLVE_THREADS) {
+        for (int c = threadIdx.x; c < m; c += T) {
             a.warm_p_next[c] = a.warm_cp[c];
             for (int q = 0; q < 3; ++q) a.warm_s_next[3 * c + q] = a.warm_sums[3 * c + q];
         }
         if (threadIdx.x == 0) *a.warm_n_next = m;
     }
-    for (int i = threadIdx.x; i < a.n; i += SOLVE_THREADS) {
+    for (int i = threadIdx.x; i < a.n; i += T) {
         if (a.k.quat) integrate_velocities_quat(a.B, a.B, i, a.k.dt);
         else integrate_velocities_body(a.B, i, a.k.dt);
     }
     if (a.rec_out) {
         __syncthreads();
-        for (int c = threadIdx.x; c < ncon && c < a.rec_cap; c += SOLVE_THREADS) a.rec_out[c] = a.con[c];
+        for (int c = threadIdx.x; c < ncon && c < a.rec_cap; c += T) a.rec_out[c] = a.con[c];
         if (threadIdx.x == 0) *a.rec_count = ncon;
     }
 }
@@ -1375,6 +1386,10 @@ int nans_upload_worlds(nans_ctx *ctx, int32_t nworlds, const int32_t *world_offs
     for (const void *f : {(const void *)k_plain_solve<false>, (const void *)k_plain_solve<true>})
         NANS_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)(sizeof(int) * PLAIN_MAX_BODIES)));
+    for (const void *f : {(const void *)k_plain_solve<false, PLAIN_SMALL_THREADS>,
+                          (const void *)k_plain_solve<true, PLAIN_SMALL_THREADS>})
+        NANS_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)(sizeof(int) * PLAIN_SMALL_BODIES + sizeof(ContactTerms) * PLAIN_SMALL_TERMS)));
     k_zero_u64<<<1, 64>>>(W->cnt, CNT_N);
     NANS_HIP(hipGetLastError());
     NANS_TRY(setup_floor(ctx, nullptr));
@@ -1594,6 +1609,10 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
             a.rec_count = W->record ? W->rec_count + w : nullptr;
             a.cnt = W->cnt;
             a.cand = cand;
+            // worlds of at most PLAIN_SMALL_BODIES bodies: 256 threads, contact terms in LDS
+            const bool small = n <= PLAIN_SMALL_BODIES;
+            a.npre = small ? PLAIN_SMALL_TERMS : 0;
+            const size_t lds = sizeof(int) * (size_t)((n + 3) & ~3) + sizeof(ContactTerms) * (size_t)a.npre;
             if (ctx->warm) {
                 const int cur = W->warm_flip, nxt = cur ^ 1;
                 const int64_t off = W->h_warm_off[w];
@@ -1606,9 +1625,11 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
                 a.warm_cap = W->h_warm_cap[w];
                 a.warm_cp = L ? L->warm_cp : W->warm_cp;
                 a.warm_sums = L ? L->warm_sums : W->warm_sums;
-                k_plain_solve<true><<<1, SOLVE_THREADS, sizeof(int) * n, ls>>>(a);
+                if (small) k_plain_solve<true, PLAIN_SMALL_THREADS><<<1, PLAIN_SMALL_THREADS, lds, ls>>>(a);
+                else k_plain_solve<true><<<1, SOLVE_THREADS, lds, ls>>>(a);
             } else {
-                k_plain_solve<false><<<1, SOLVE_THREADS, sizeof(int) * n, ls>>>(a);
+                if (small) k_plain_solve<false, PLAIN_SMALL_THREADS><<<1, PLAIN_SMALL_THREADS, lds, ls>>>(a);
+                else k_plain_solve<false><<<1, SOLVE_THREADS, lds, ls>>>(a);
             }
             NANS_HIP(hipGetLastError());
         }

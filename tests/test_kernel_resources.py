@@ -1,7 +1,7 @@
-"""Register budget of the batched step kernel, from the compiler's resource
-report for gfx950 (no GPU needed): every instantiation (64-512 threads per
-world, cold and warm-started) keeps two waves per SIMD with no VGPR spills and
-no scratch memory.  A spill there costs the step its speed (a warm-started
+"""Register budget of the step kernels, from the compiler's resource report for
+gfx950 (no GPU needed): every instantiation of the batched step (64-512 threads
+per world, cold and warm-started) keeps two waves per SIMD with no VGPR spills
+and no scratch memory, and so does every variant of the plain step's solve.  A spill there costs the step its speed (a warm-started
 variant that spilled 67 registers ran at half the speed of the cold one)."""
 import os
 import re
@@ -42,3 +42,9 @@ def test_batched_step_has_no_spills():
         assert v.get("VGPRs Spill") == 0, (k, v)
         assert v.get("ScratchSize [bytes/lane]") == 0, (k, v)
         assert v.get("Occupancy [waves/SIMD]", 0) >= 2, (k, v)
+    # the plain step's solve: 1024 threads (large worlds) and 256 with contact terms in LDS
+    solve = {k: v for k, v in kern.items() if "k_plain_solve" in k}
+    assert len(solve) == 4, sorted(kern)
+    for k, v in solve.items():
+        assert v.get("VGPRs Spill") == 0, (k, v)
+        assert v.get("ScratchSize [bytes/lane]") == 0, (k, v)
