@@ -1,8 +1,10 @@
 """Config 5 (one world of 8192 boxes dropped on the ground box) through the plain
 step: a reduced block of the grid bit-exact with the oracle for 40 steps, the
 full world bit-exact after its first step (33.5M canonical pair tests), and the
-full 480-step run checked for finite state, no capacity overflow and boxes
-resting above the ground."""
+full 480-step run checked for finite state, no capacity overflow and the
+oracle's golden state hash.  The reference's rules launch some boxes far from
+the pile in that run (the effective-mass term of Constraint, see
+test_floor_degeneracy.py), so no resting position is asserted."""
 import numpy as np
 import pytest
 
