@@ -701,10 +701,9 @@ struct EpgLds {
     float sf[64 / G][G][9];   // compaction staging: n, a, d, id, cl
     int ekey[64];             // horizon edges per group (G each)
     int eids[64];
-    // horizon: bit ca * G + cb of the directed edges (ca, cb) of the visible
-    // faces, set once seen (m1) and once seen twice (m2)
-    uint32_t m1[64 / G][G * G / 32];
-    uint32_t m2[64 / G][G * G / 32];
+    // horizon: how often the visible faces push each directed edge (ca, cb),
+    // one byte per key ca * G + cb, four keys per word
+    uint32_t ecnt[64 / G][G * G / 4];
 };
 static_assert(sizeof(EpgLds<16>) <= sizeof(EpaLds), "the grouped EPA reuses the wave's EPA LDS");
 static_assert(sizeof(EpgLds<32>) <= sizeof(EpaLds), "the grouped EPA reuses the wave's EPA LDS");
@@ -929,13 +928,13 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
     V3 cN = z;
     float ccur = 0.0f, cd0 = 0.0f;
     int cfid = 0, cbest = 0;
-    // the horizon's edge maps (below), clear at the start of every trip
-    constexpr int MW = G * G / 32;
-    static_assert(MW <= G, "a group's lanes clear its edge maps");
-    if (j < MW) {
-        L.m1[g][j] = 0u;
-        L.m2[g][j] = 0u;
-    }
+    // the horizon's edge counts (below), zero at the start of every trip
+    constexpr int CW = G * G / 4;  // words of counts per group, G / 4 per lane
+    auto clear_counts = [&]() {
+#pragma unroll
+        for (int k = 0; k < CW / G; ++k) L.ecnt[g][j * (CW / G) + k] = 0u;
+    };
+    clear_counts();
     wave_sync();
     for (;;) {
         // while (iter++ <= MAX_EPA_ITERATIONS), per group
@@ -994,12 +993,12 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
         const int nvis = __popc(vis);
         // horizon: edges of the visible faces in face order; an edge survives iff its
         // reverse is not among them, when no key is pushed twice (else spill).  The
-        // visible faces' directed edges between non-NaN classes (< G) go into the
-        // group's edge maps; bits of this lane's three edges: same key seen twice
-        // (s2), reverse key seen (r1) and seen twice (r2).  Edges with a NaN class
-        // never match, as vertex == never holds for them.
-        // The maps are clear here: cleared before the loop and after each trip's
-        // last read of them.
+        // visible faces' directed edges between non-NaN classes (< G) are counted
+        // per key (at most 3G pushes, so a byte never wraps); bits of this lane's
+        // three edges: same key pushed twice (s2), reverse key pushed (r1) and
+        // pushed twice (r2).  Edges with a NaN class never match, as vertex ==
+        // never holds for them.  The counts are zero here: cleared before the loop
+        // and after each trip's last read of them.
         const int cl = F.cl;
         if (w0) {
 #pragma unroll
@@ -1008,8 +1007,7 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
                 const int ca = ke & 255, cb = ke >> 8;
                 if (ca < G && cb < G) {  // classes are < G, or CLS_NAN
                     const int key = ca * G + cb;
-                    const uint32_t bit = 1u << (key & 31);
-                    if (atomicOr(&L.m1[g][key >> 5], bit) & bit) atomicOr(&L.m2[g][key >> 5], bit);
+                    atomicAdd(&L.ecnt[g][key >> 2], 1u << (8 * (key & 3)));
                 }
             }
         }
@@ -1022,9 +1020,11 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
                 const int ca = ke & 255, cb = ke >> 8;
                 if (ca < G && cb < G) {
                     const int key = ca * G + cb, rkey = cb * G + ca;
-                    s2 |= ((L.m2[g][key >> 5] >> (key & 31)) & 1u) << e;
-                    r1 |= ((L.m1[g][rkey >> 5] >> (rkey & 31)) & 1u) << e;
-                    r2 |= ((L.m2[g][rkey >> 5] >> (rkey & 31)) & 1u) << e;
+                    const uint32_t nk = (L.ecnt[g][key >> 2] >> (8 * (key & 3))) & 255u;
+                    const uint32_t nr = (L.ecnt[g][rkey >> 2] >> (8 * (rkey & 3))) & 255u;
+                    s2 |= (nk >= 2u ? 1u : 0u) << e;
+                    r1 |= (nr >= 1u ? 1u : 0u) << e;
+                    r2 |= (nr >= 2u ? 1u : 0u) << e;
                 }
             }
         }
@@ -1057,10 +1057,7 @@ __device__ inline EpaResult epa_group(EpgLds<G> &L, const EpaShapes &sh, V3 sa0,
             }
         }
         wave_sync();
-        if (j < MW) {  // every read of the maps is behind the sync above
-            L.m1[g][j] = 0u;
-            L.m2[g][j] = 0u;
-        }
+        clear_counts();  // every read of the counts is behind the sync above
         int ea = 0, eb = 0, eca = 0, ecb = 0;
         if (run && j < ne) {
             const int k2 = L.ekey[gb + j], i2 = L.eids[gb + j];
