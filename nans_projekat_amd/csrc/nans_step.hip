@@ -28,6 +28,7 @@
 #include <cmath>
 #include <map>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 using namespace nans;
@@ -624,7 +625,7 @@ struct WorldScratch {
 
 // The whole step of world w by team t (the workgroup, or one wave of it): epa[k]
 // is the polytope LDS of the team's wave k, wave_tot one int per wave.
-template <int BLOCK, bool WARM>
+template <int BLOCK, bool WARM, bool ISL>
 __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const Team &t, int w, const WorldLds &L,
                                               const WorldScratch &S, EpaLds *epa, NarrowShared &nsh,
                                               int *wave_tot, const FloorDev &fl)
@@ -761,7 +762,7 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
     // narrowphase over the candidates (GJK per lane, EPA per wave)
     {
         WorldSrc<true> src{Lb, vertsL, nb, ns, &fl, cand, rec, flag, a.cnt};
-        narrow_team<false>(t, src, ncand, a.k.np, epa, nsh, S.qidx, S.qsim);
+        narrow_team<false, ISL>(t, src, ncand, a.k.np, epa, nsh, S.qidx, S.qsim);
     }
     if (tid == 0) {
         atomicAdd(&a.cnt[CNT_GJK_FOUND], (unsigned long long)nsh.nepa);
@@ -847,7 +848,9 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
 // from its own counter (queue[0], queue[1]).
 // WARM: the opt-in warm start, each contact's accumulators carried over from the
 // world's previous step (a separate instantiation; the default step has none of it).
-template <int BLOCK, bool WARM = false>
+// ISL: the opt-in FP64-islands mode, EPA in the grouped tiers with a binary64 final
+// face (a separate instantiation, so that the default step carries no binary64 code).
+template <int BLOCK, bool WARM = false, bool ISL = false>
 __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
 {
     extern __shared__ float smem[];
@@ -890,7 +893,7 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
             job = a.nbig + __builtin_amdgcn_readfirstlane(j);
         }
         if (job >= a.nworlds) break;
-        batched_world<BLOCK, WARM>(a, t, a.order[job], L, S, tepa, nsh[tsh], wave_tot, fl);
+        batched_world<BLOCK, WARM, ISL>(a, t, a.order[job], L, S, tepa, nsh[tsh], wave_tot, fl);
     }
 }
 
@@ -914,12 +917,12 @@ namespace nans {
 
 void wfree(void *p)
 {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
 }
 
 static void drop_graph(nans_worlds *W)
 {
-    if (W->graph) hipGraphExecDestroy(W->graph);
+    if (W->graph) (void)hipGraphExecDestroy(W->graph);
     W->graph = nullptr;
     W->graph_steps = -1;
 }
@@ -961,11 +964,11 @@ static void release_plain_lanes(nans_worlds *W)
                         (void *)L.qidx, (void *)L.qsim, (void *)L.blk_counts, (void *)L.blk_offsets,
                         (void *)L.ncand, (void *)L.warm_cp, (void *)L.warm_sums})
             wfree(q);
-        if (L.ev) hipEventDestroy(L.ev);
-        if (L.s) hipStreamDestroy(L.s);
+        if (L.ev) (void)hipEventDestroy(L.ev);
+        if (L.s) (void)hipStreamDestroy(L.s);
     }
     W->plain_lanes.clear();
-    if (W->plain_ev0) hipEventDestroy(W->plain_ev0);
+    if (W->plain_ev0) (void)hipEventDestroy(W->plain_ev0);
     W->plain_ev0 = nullptr;
 }
 
@@ -986,7 +989,7 @@ int worlds_release(nans_ctx *ctx)
                     (void *)W->warm_sums, (void *)W->warm_offd, (void *)W->warm_capd, (void *)W->snap})
         wfree(p);
     for (hipEvent_t ev : {W->ev_in, W->ev_out, W->ev_t0, W->ev_t1})
-        if (ev) hipEventDestroy(ev);
+        if (ev) (void)hipEventDestroy(ev);
     release_batch_scratch(W);
     release_recording(W);
     release_plain_lanes(W);
@@ -1648,15 +1651,29 @@ int nans_step_plain(nans_ctx *ctx, int32_t nsteps)
 
 namespace nans {
 
-template <int BLOCK, bool WARM = false>
+// f(B, I) with B the workgroup size (64, 128, 256 or 512) and I the FP64-islands
+// flag as compile-time constants: picks the batched kernel's instantiation.
+template <class F>
+static int by_shape(int block, bool isl, F &&f)
+{
+    auto go = [&](auto B) { return isl ? f(B, std::true_type{}) : f(B, std::false_type{}); };
+    switch (block) {
+    case 64: return go(std::integral_constant<int, 64>{});
+    case 128: return go(std::integral_constant<int, 128>{});
+    case 512: return go(std::integral_constant<int, 512>{});
+    default: return go(std::integral_constant<int, 256>{});
+    }
+}
+
+template <int BLOCK, bool WARM, bool ISL>
 static int batched_config(nans_ctx *ctx, const WaveSplit &sp, int *grid, size_t *lds)
 {
     nans_worlds *W = ctx->worlds;
     *lds = batch_lds_bytes(W->nmax, sp, BLOCK);
-    NANS_HIP(hipFuncSetAttribute((const void *)k_batched_step<BLOCK, WARM>,
+    NANS_HIP(hipFuncSetAttribute((const void *)k_batched_step<BLOCK, WARM, ISL>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)*lds));
     int per_cu = 0;
-    NANS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batched_step<BLOCK, WARM>, BLOCK, *lds));
+    NANS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batched_step<BLOCK, WARM, ISL>, BLOCK, *lds));
     NANS_REQUIRE(per_cu > 0, NANS_ERR_CAPACITY, "a world of %d bodies does not fit the LDS of one workgroup",
                  W->nmax);
     int cus = 0;
@@ -1668,7 +1685,7 @@ static int batched_config(nans_ctx *ctx, const WaveSplit &sp, int *grid, size_t 
 }
 
 // Launch shape, per-workgroup scratch and kernel arguments of the batched step.
-template <int BLOCK, bool WARM = false>
+template <int BLOCK, bool WARM, bool ISL>
 static int batched_prepare(nans_ctx *ctx, BatchArgs &a, int *grid_out, size_t *lds_out,
                            WaveSplit *split_out = nullptr)
 {
@@ -1676,7 +1693,7 @@ static int batched_prepare(nans_ctx *ctx, BatchArgs &a, int *grid_out, size_t *l
     int grid = 0;
     size_t lds = 0;
     const WaveSplit sp = wave_split(W, wave_bodies(W, BLOCK));
-    NANS_TRY((batched_config<BLOCK, WARM>(ctx, sp, &grid, &lds)));
+    NANS_TRY((batched_config<BLOCK, WARM, ISL>(ctx, sp, &grid, &lds)));
     constexpr int NW = BLOCK / 64;
     int64_t cap = std::min<int64_t>(W->max_pairs, 32LL * W->nmax + 1024);
     cap = std::max<int64_t>(cap, 64);
@@ -1732,7 +1749,7 @@ static int batched_prepare(nans_ctx *ctx, BatchArgs &a, int *grid_out, size_t *l
     return NANS_OK;
 }
 
-template <int BLOCK>
+template <int BLOCK, bool ISL>
 static int batched_capture(nans_ctx *ctx, int nsteps, hipStream_t cs)
 {
     nans_worlds *W = ctx->worlds;
@@ -1740,18 +1757,18 @@ static int batched_capture(nans_ctx *ctx, int nsteps, hipStream_t cs)
     int grid = 0;
     size_t lds = 0;
     WaveSplit sp;
-    NANS_TRY(batched_prepare<BLOCK>(ctx, a, &grid, &lds, &sp));
+    NANS_TRY((batched_prepare<BLOCK, false, ISL>(ctx, a, &grid, &lds, &sp)));
     // one step: reset the work counters with a fill kernel (no memset node), then the
     // persistent step kernel; a call of nsteps replays this graph nsteps times
     hipGraph_t g = nullptr;
     NANS_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     k_zero_i32<<<1, 64, 0, cs>>>(W->queue, 2);
-    k_batched_step<BLOCK><<<grid, BLOCK, lds, cs>>>(a);
+    k_batched_step<BLOCK, false, ISL><<<grid, BLOCK, lds, cs>>>(a);
     hipError_t e = hipStreamEndCapture(cs, &g);
     NANS_REQUIRE(e == hipSuccess, NANS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
     drop_graph(W);
     e = hipGraphInstantiate(&W->graph, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     NANS_REQUIRE(e == hipSuccess, NANS_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
     W->graph_steps = nsteps;
     W->graph_block = BLOCK;
@@ -1783,12 +1800,9 @@ int batched_run(nans_ctx *ctx, int nsteps, int block)
     if (!reuse) {
         NANS_TRY(setup_floor(ctx, ctx->stream));
         NANS_HIP(hipStreamSynchronize(ctx->stream));
-        switch (block) {
-        case 64: NANS_TRY(batched_capture<64>(ctx, 1, cs)); break;
-        case 128: NANS_TRY(batched_capture<128>(ctx, 1, cs)); break;
-        case 512: NANS_TRY(batched_capture<512>(ctx, 1, cs)); break;
-        default: NANS_TRY(batched_capture<256>(ctx, 1, cs)); break;
-        }
+        NANS_TRY(by_shape(block, ctx->p.fp64_islands != 0, [&](auto B, auto I) {
+            return batched_capture<decltype(B)::value, decltype(I)::value>(ctx, 1, cs);
+        }));
     }
     NANS_HIP(hipEventRecord(W->ev_in, ctx->stream));
     NANS_HIP(hipStreamWaitEvent(cs, W->ev_in, 0));
@@ -1804,7 +1818,7 @@ int batched_run(nans_ctx *ctx, int nsteps, int block)
 
 // The warm-started batched step: launched step by step (no graph), since each
 // step swaps the previous and next contact lists.
-template <int BLOCK>
+template <int BLOCK, bool ISL>
 static int batched_warm_launches(nans_ctx *ctx, int nsteps, hipStream_t cs)
 {
     nans_worlds *W = ctx->worlds;
@@ -1812,7 +1826,7 @@ static int batched_warm_launches(nans_ctx *ctx, int nsteps, hipStream_t cs)
     int grid = 0;
     size_t lds = 0;
     WaveSplit sp;
-    NANS_TRY((batched_prepare<BLOCK, true>(ctx, a, &grid, &lds, &sp)));
+    NANS_TRY((batched_prepare<BLOCK, true, ISL>(ctx, a, &grid, &lds, &sp)));
     a.warm_off = W->warm_offd;
     a.warm_cap = W->warm_capd;
     for (int s = 0; s < nsteps; ++s) {
@@ -1824,7 +1838,7 @@ static int batched_warm_launches(nans_ctx *ctx, int nsteps, hipStream_t cs)
         a.warm_s_next = W->warm_s[nxt];
         a.warm_n_next = W->warm_n[nxt];
         k_zero_i32<<<1, 64, 0, cs>>>(a.queue, 2);
-        k_batched_step<BLOCK, true><<<grid, BLOCK, lds, cs>>>(a);
+        k_batched_step<BLOCK, true, ISL><<<grid, BLOCK, lds, cs>>>(a);
         NANS_HIP(hipGetLastError());
         W->warm_flip = nxt;
     }
@@ -1855,12 +1869,9 @@ int batched_run_warm(nans_ctx *ctx, int nsteps, int block)
     NANS_HIP(hipEventRecord(W->ev_in, ctx->stream));
     NANS_HIP(hipStreamWaitEvent(cs, W->ev_in, 0));
     NANS_HIP(hipEventRecord(W->ev_t0, cs));
-    switch (block) {
-    case 64: NANS_TRY(batched_warm_launches<64>(ctx, nsteps, cs)); break;
-    case 128: NANS_TRY(batched_warm_launches<128>(ctx, nsteps, cs)); break;
-    case 512: NANS_TRY(batched_warm_launches<512>(ctx, nsteps, cs)); break;
-    default: NANS_TRY(batched_warm_launches<256>(ctx, nsteps, cs)); break;
-    }
+    NANS_TRY(by_shape(block, ctx->p.fp64_islands != 0, [&](auto B, auto I) {
+        return batched_warm_launches<decltype(B)::value, decltype(I)::value>(ctx, nsteps, cs);
+    }));
     NANS_HIP(hipEventRecord(W->ev_t1, cs));
     NANS_HIP(hipEventRecord(W->ev_out, cs));
     NANS_HIP(hipStreamWaitEvent(ctx->stream, W->ev_out, 0));
@@ -1878,20 +1889,20 @@ int batched_block(nans_ctx *ctx)
     return W->nmax <= 64 ? 128 : 256;
 }
 
-template <int BLOCK>
+template <int BLOCK, bool ISL>
 static int batched_recorded_launch(nans_ctx *ctx, hipStream_t s, ContactRec *rec_out, const int64_t *rec_off,
                                    int *rec_count, const int *rec_cap)
 {
     BatchArgsT<false> a;
     int grid = 0;
     size_t lds = 0;
-    NANS_TRY(batched_prepare<BLOCK>(ctx, a, &grid, &lds));
+    NANS_TRY((batched_prepare<BLOCK, false, ISL>(ctx, a, &grid, &lds)));
     a.rec_out = rec_out;
     a.rec_off = rec_off;
     a.rec_count = rec_count;
     a.rec_cap = rec_cap;
     k_zero_i32<<<1, 64, 0, s>>>(a.queue, 2);
-    k_batched_step<BLOCK><<<grid, BLOCK, lds, s>>>(a);
+    k_batched_step<BLOCK, false, ISL><<<grid, BLOCK, lds, s>>>(a);
     NANS_HIP(hipGetLastError());
     return NANS_OK;
 }
@@ -1903,12 +1914,10 @@ int batched_step_recorded(nans_ctx *ctx, hipStream_t s, ContactRec *rec_out, con
     NANS_REQUIRE(W->nmax <= BATCH_MAX_BODIES, NANS_ERR_CAPACITY,
                  "the batched step keeps a world in LDS: at most %d bodies per world (largest: %d)",
                  BATCH_MAX_BODIES, W->nmax);
-    switch (batched_block(ctx)) {
-    case 64: return batched_recorded_launch<64>(ctx, s, rec_out, rec_off, rec_count, rec_cap);
-    case 128: return batched_recorded_launch<128>(ctx, s, rec_out, rec_off, rec_count, rec_cap);
-    case 512: return batched_recorded_launch<512>(ctx, s, rec_out, rec_off, rec_count, rec_cap);
-    default: return batched_recorded_launch<256>(ctx, s, rec_out, rec_off, rec_count, rec_cap);
-    }
+    return by_shape(batched_block(ctx), ctx->p.fp64_islands != 0, [&](auto B, auto I) {
+        return batched_recorded_launch<decltype(B)::value, decltype(I)::value>(ctx, s, rec_out, rec_off, rec_count,
+                                                                              rec_cap);
+    });
 }
 
 }  // namespace nans
