@@ -56,6 +56,8 @@ struct BatchArgs {
     SimplexRec *qsim;
     int cap;                // entries per workgroup
     int wcap;               // entries per wave in the wave phase (cap / waves)
+    float *corners;         // per-workgroup box corners, corner_stride floats each
+    int64_t corner_stride;
     ContactRec *rec_out;    // recording (may be null)
     const int64_t *rec_off;
     int *rec_count;
@@ -199,6 +201,8 @@ struct nans_worlds {
     int *b_flag = nullptr;
     int *b_qidx = nullptr;
     nans::SimplexRec *b_qsim = nullptr;
+    float *b_corners = nullptr;    // box corners of the batched step's worlds in flight
+    int64_t b_corner_floats = 0;   // its size
     int *queue = nullptr;
     int queue_len = 0;
     // recording

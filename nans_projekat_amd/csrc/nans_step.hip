@@ -585,28 +585,35 @@ __global__ __launch_bounds__(T) void k_plain_solve(SolveArgs a)
 
 // ------------------------------------------------------------------ batched kernel
 
-// Floats of one world's LDS for up to rows bodies (F_NLDS planes, 24 corner
-// coordinates, the bounding radius and the solver's head per body).
-__host__ __device__ constexpr int world_lds_floats(int rows) { return (F_NLDS + 26) * rows; }
+// Floats of one world's LDS for up to rows bodies (F_NLDS planes, the bounding
+// radius and the solver's head per body).
+__host__ __device__ constexpr int world_lds_floats(int rows) { return (F_NLDS + 2) * rows; }
 
-// One world's LDS in the batched step: body planes, the solver's heads, bounding
-// radii and box corners, for up to rows bodies.  The radii and corners (idle
-// once the narrowphase ends) are adjacent: the solver keeps contact terms there.
+// Floats of one world's box corners (global memory) for up to rows bodies.
+__host__ __device__ constexpr int64_t world_corner_floats(int rows) { return 24LL * rows; }
+
+// One world's memory in the batched step: body planes, the solver's heads and
+// bounding radii in LDS for up to rows bodies, and the box corners in the
+// workgroup's global scratch.  The corners stay out of LDS so that a world of
+// 256 bodies fits three workgroups to a CU (three waves per SIMD); the GJK
+// lanes hold a pair's corners in registers, and the rest of their reads hit
+// the L1/L2.  The radii are idle once the narrowphase ends: the solver keeps
+// contact terms there.
 struct WorldLds {
-    float *body;   // [F_NLDS][rows]
-    int *head;     // [rows]
-    float *brad;   // [rows]
-    float *verts;  // [rows][24]
+    float *body;   // [F_NLDS][rows] (LDS)
+    int *head;     // [rows] (LDS)
+    float *brad;   // [rows] (LDS)
+    float *verts;  // [rows][24] (global)
     int rows;
 };
 
-__device__ inline WorldLds world_lds(float *base, int rows)
+__device__ inline WorldLds world_lds(float *base, int rows, float *corners)
 {
     WorldLds L;
     L.body = base;
     L.head = reinterpret_cast<int *>(base + F_NLDS * rows);
     L.brad = base + (F_NLDS + 1) * rows;
-    L.verts = L.brad + rows;
+    L.verts = corners;
     L.rows = rows;
     return L;
 }
@@ -810,10 +817,10 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
     }
     const float Cf = a.wfric ? a.wfric[w] : a.k.friction;
     const float Cr = a.wrest ? a.wrest[w] : a.k.restitution;
-    // contact terms in the radii and corners, then in the team's polytope LDS
+    // contact terms in the radii, then in the team's polytope LDS
     PreTerms pre;
     pre.a = reinterpret_cast<ContactTerms *>(L.brad);
-    pre.na = (int)(sizeof(float) * 25 * L.rows / sizeof(ContactTerms));
+    pre.na = (int)(sizeof(float) * L.rows / sizeof(ContactTerms));
     pre.b = reinterpret_cast<ContactTerms *>(epa);
     pre.nb = (int)(NW * sizeof(EpaLds) / sizeof(ContactTerms));
     solve_team<WARM>(t, Lb, n, DirectCon{con}, ncon, done, L.head, fl, a.k, Cf, Cr, pre, WARM ? wsum : nullptr);
@@ -850,22 +857,28 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
 // world's previous step (a separate instantiation; the default step has none of it).
 // ISL: the opt-in FP64-islands mode, EPA in the grouped tiers with a binary64 final
 // face (a separate instantiation, so that the default step carries no binary64 code).
+// Registers: three waves per SIMD (168 VGPRs; the compiler spills a few dozen,
+// which the third world per CU more than repays), two at 512 threads, whose
+// workgroup fits a CU once.
+__host__ __device__ constexpr int batch_waves_per_simd(int block) { return block >= 512 ? 2 : 3; }
+
 template <int BLOCK, bool WARM = false, bool ISL = false>
-__global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(batch_waves_per_simd(BLOCK)))) void
+k_batched_step(BatchArgsT<WARM> a)
 {
     extern __shared__ float smem[];
     constexpr int NW = BLOCK / 64;
     __shared__ int wave_tot[NW];
     __shared__ int s_job;
-    __shared__ FloorDev fl;
+    const FloorDev &fl = *a.fl;  // in global memory, as the box corners
     __shared__ NarrowShared nsh[NW];
-    if (threadIdx.x == 0) fl = *a.fl;
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     EpaLds *const epa = reinterpret_cast<EpaLds *>(smem + a.lds_floats);
     const int64_t wg = (int64_t)blockIdx.x * a.cap;
     // the workgroup's team, LDS and scratch until the first phase ends
     Team t = Team::block(BLOCK);
-    WorldLds L = world_lds(smem, a.nmax);
+    float *const corners = a.corners + (int64_t)blockIdx.x * a.corner_stride;
+    WorldLds L = world_lds(smem, a.nmax, corners);
     WorldScratch S{a.cand + wg, a.rec + wg, a.flag + wg, a.qidx + wg, a.qsim + wg, a.cap};
     EpaLds *tepa = epa;
     int tsh = 0;
@@ -880,7 +893,8 @@ __global__ __launch_bounds__(BLOCK, 2) void k_batched_step(BatchArgsT<WARM> a)
             if (job >= a.nbig) {  // the wave phase: each wave its own LDS part and scratch
                 whole = false;
                 t = Team::wave();
-                L = world_lds(smem + wv * world_lds_floats(a.wrows), a.wrows);
+                L = world_lds(smem + wv * world_lds_floats(a.wrows), a.wrows,
+                              corners + wv * world_corner_floats(a.wrows));
                 const int64_t o = wg + (int64_t)wv * a.wcap;
                 S = WorldScratch{a.cand + o, a.rec + o, a.flag + o, a.qidx + o, a.qsim + o, a.wcap};
                 tepa = epa + wv;
@@ -937,11 +951,14 @@ static void release_batch_scratch(nans_worlds *W)
     wfree(W->b_flag);
     wfree(W->b_qidx);
     wfree(W->b_qsim);
+    wfree(W->b_corners);
     W->b_cand = nullptr;
     W->b_rec = nullptr;
     W->b_flag = nullptr;
     W->b_qidx = nullptr;
     W->b_qsim = nullptr;
+    W->b_corners = nullptr;
+    W->b_corner_floats = 0;
     W->b_grid = 0;
 }
 
@@ -1737,6 +1754,17 @@ static int batched_prepare(nans_ctx *ctx, BatchArgs &a, int *grid_out, size_t *l
     a.qsim = W->b_qsim;
     a.cap = W->b_cap;
     a.wcap = W->b_cap / NW;
+    // box corners: a workgroup world's, or its waves' worlds side by side
+    a.corner_stride = std::max(world_corner_floats(W->nmax), NW * world_corner_floats(sp.wrows));
+    if (W->b_corner_floats < a.corner_stride * grid) {
+        drop_graph(W);
+        wfree(W->b_corners);
+        W->b_corners = nullptr;
+        W->b_corner_floats = 0;
+        NANS_TRY(dmalloc(&W->b_corners, (size_t)(a.corner_stride * grid)));
+        W->b_corner_floats = a.corner_stride * grid;
+    }
+    a.corners = W->b_corners;
     a.rec_out = W->record ? W->rec_buf : nullptr;
     a.rec_off = W->rec_off;
     a.rec_count = W->rec_count;

@@ -1,8 +1,11 @@
 """Register budget of the step kernels, from the compiler's resource report for
-gfx950 (no GPU needed): every instantiation of the batched step (64-512 threads
-per world, cold and warm-started, reference and FP64-islands arithmetic) keeps two waves per SIMD with no VGPR spills
-and no scratch memory, and so does every variant of the plain step's solve.  A spill there costs the step its speed (a warm-started
-variant that spilled 67 registers ran at half the speed of the cold one)."""
+gfx950 (no GPU needed).  The batched step (64-512 threads per world, cold and
+warm-started, reference and FP64-islands arithmetic) is compiled for three waves
+per SIMD up to 256 threads (168 VGPRs, so that three worlds share a CU) and two
+at 512: the test holds each instantiation to its occupancy and its spills to the
+few dozen registers that budget costs (a warm-started variant that once spilled
+67 registers at the two-wave budget ran at half the speed of the cold one).
+Every variant of the plain step's solve keeps its registers without spills."""
 import os
 import re
 import subprocess
@@ -39,9 +42,16 @@ def test_batched_step_has_no_spills():
     batched = {k: v for k, v in kern.items() if "k_batched_step" in k}
     assert len(batched) == 16, sorted(kern)
     for k, v in batched.items():
-        assert v.get("VGPRs Spill") == 0, (k, v)
-        assert v.get("ScratchSize [bytes/lane]") == 0, (k, v)
-        assert v.get("Occupancy [waves/SIMD]", 0) >= 2, (k, v)
+        block = int(re.search(r"k_batched_stepILi(\d+)E", k).group(1))
+        if block >= 512:
+            assert v.get("Occupancy [waves/SIMD]", 0) >= 2, (k, v)
+            assert v.get("VGPRs Spill") == 0, (k, v)
+            assert v.get("ScratchSize [bytes/lane]") == 0, (k, v)
+        else:
+            assert v.get("Occupancy [waves/SIMD]", 0) >= 3, (k, v)
+            # the default (cold, reference arithmetic) step at 128 and 256 threads
+            cold = "Lb0ELb0E" in k and block >= 128
+            assert v.get("VGPRs Spill") <= (40 if cold else 96), (k, v)
     # the plain step's solve: 1024 threads (large worlds) and 256 with contact terms in LDS
     solve = {k: v for k, v in kern.items() if "k_plain_solve" in k}
     assert len(solve) == 4, sorted(kern)
