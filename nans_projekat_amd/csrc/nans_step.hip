@@ -857,10 +857,12 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
 // world's previous step (a separate instantiation; the default step has none of it).
 // ISL: the opt-in FP64-islands mode, EPA in the grouped tiers with a binary64 final
 // face (a separate instantiation, so that the default step carries no binary64 code).
-// Registers: three waves per SIMD (168 VGPRs; the compiler spills a few dozen,
-// which the third world per CU more than repays), two at 512 threads, whose
-// workgroup fits a CU once.
-__host__ __device__ constexpr int batch_waves_per_simd(int block) { return block >= 512 ? 2 : 3; }
+// Registers: at 256 threads three waves per SIMD (168 VGPRs; the compiler
+// spills a few dozen, which a third world per CU more than repays); two at 64
+// and 128 threads, the sizes for batches of small worlds that fit the chip at
+// once (a step then lasts one world's step, which the spills would lengthen),
+// and at 512, whose workgroup fits a CU once.
+__host__ __device__ constexpr int batch_waves_per_simd(int block) { return block == 256 ? 3 : 2; }
 
 template <int BLOCK, bool WARM = false, bool ISL = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(batch_waves_per_simd(BLOCK)))) void
