@@ -189,9 +189,12 @@ struct WorldSrc {
 
 // Dependency rounds of SolveConstraints over con[0..ncon) (see the file comment).
 // head[] has room for n ints; done[] for ncon.  The body slots of each thread's
-// first SOLVE_REG contacts stay in registers for all rounds; contacts beyond
-// SOLVE_REG * BLOCK are read from con[] each round.
+// first REG contacts stay in registers for all rounds; contacts beyond REG * BLOCK
+// are read from con[] each round.  The plain step's solve keeps SOLVE_REG; the
+// batched step, compiled for three waves per SIMD, keeps one (two cost it 2.5 %
+// of its step in spilled registers).
 constexpr int SOLVE_REG = 2;
+constexpr int BATCH_SOLVE_REG = 1;
 
 // Contact c of a world's ordered list.
 struct DirectCon {
@@ -213,7 +216,7 @@ struct PreTerms {
 // pre receives the terms of the first contacts in one parallel pass (as many as
 // it holds), so that the rounds only evaluate the velocity-dependent part for
 // them.  con(c) is contact c's record (DirectCon).
-template <bool WARM = false, class Con>
+template <bool WARM = false, int REG = SOLVE_REG, class Con>
 __device__ inline void solve_team(const Team &t, const Planes &B, int n, const Con &con, int ncon, int *done,
                                   int *head, const FloorDev &fl, const StepConsts &k, float Cf, float Cr,
                                   const PreTerms &pre = PreTerms{}, float *warm = nullptr)
@@ -221,21 +224,21 @@ __device__ inline void solve_team(const Team &t, const Planes &B, int n, const C
     const int tid = t.rank, BLOCK = t.size;
     const int npre = min(pre.na + pre.nb, ncon);
     for (int c = tid; c < npre; c += BLOCK) contact_terms(B, con(c), fl, k, pre(c));
-    // a thread's first SOLVE_REG contacts live in registers: their bodies, and
+    // a thread's first REG contacts live in registers: their bodies, and
     // the impulses they apply, written back once after the rounds (no global
     // memory access on a round's critical path when their terms are in pre[])
-    int sa[SOLVE_REG], sb[SOLVE_REG];
-    bool pend[SOLVE_REG];
-    float dl[SOLVE_REG][3];
+    int sa[REG], sb[REG];
+    bool pend[REG];
+    float dl[REG][3];
 #pragma unroll
-    for (int r = 0; r < SOLVE_REG; ++r) {
+    for (int r = 0; r < REG; ++r) {
         const int c = tid + r * BLOCK;
         pend[r] = c < ncon;
         sa[r] = pend[r] ? con(c).slot_a : 0;
         sb[r] = pend[r] ? con(c).slot_b : -1;
         dl[r][0] = dl[r][1] = dl[r][2] = 0.0f;
     }
-    for (int c = tid + SOLVE_REG * BLOCK; c < ncon; c += BLOCK) done[c] = 0;
+    for (int c = tid + REG * BLOCK; c < ncon; c += BLOCK) done[c] = 0;
     // Round k's key of contact c is (k << 21) | (2^21 - 1 - c) (contacts number
     // below 2^21 on both paths): the largest key a body receives in round k is its
     // earliest pending contact, and keys of earlier rounds are smaller, so head[]
@@ -254,14 +257,14 @@ __device__ inline void solve_team(const Team &t, const Planes &B, int n, const C
         const unsigned rk = (round << 21) | 0x1FFFFFu;
         auto key = [rk](int c) { return rk - (unsigned)c; };
 #pragma unroll
-        for (int r = 0; r < SOLVE_REG; ++r) {
+        for (int r = 0; r < REG; ++r) {
             if (pend[r]) {
                 const unsigned kc = key(tid + r * BLOCK);
                 atomicMax(&hk[sa[r]], kc);
                 if (sb[r] >= 0) atomicMax(&hk[sb[r]], kc);
             }
         }
-        for (int c = tid + SOLVE_REG * BLOCK; c < ncon; c += BLOCK) {
+        for (int c = tid + REG * BLOCK; c < ncon; c += BLOCK) {
             if (!done[c]) {
                 atomicMax(&hk[con(c).slot_a], key(c));
                 const int b = con(c).slot_b;
@@ -280,7 +283,7 @@ __device__ inline void solve_team(const Team &t, const Planes &B, int n, const C
             con(c).dlt2 = r.dlt2;
         };
 #pragma unroll
-        for (int r = 0; r < SOLVE_REG; ++r) {
+        for (int r = 0; r < REG; ++r) {
             if (pend[r]) {
                 const int c = tid + r * BLOCK;
                 const unsigned kc = key(c);
@@ -304,7 +307,7 @@ __device__ inline void solve_team(const Team &t, const Planes &B, int n, const C
                 }
             }
         }
-        for (int c = tid + SOLVE_REG * BLOCK; c < ncon; c += BLOCK) {
+        for (int c = tid + REG * BLOCK; c < ncon; c += BLOCK) {
             if (!done[c]) {
                 const int a = con(c).slot_a, b = con(c).slot_b;
                 if (hk[a] == key(c) && (b < 0 || hk[b] == key(c))) {
@@ -318,7 +321,7 @@ __device__ inline void solve_team(const Team &t, const Planes &B, int n, const C
         pending = t.sync_count(left);
     }
 #pragma unroll
-    for (int r = 0; r < SOLVE_REG; ++r) {
+    for (int r = 0; r < REG; ++r) {
         const int c = tid + r * BLOCK;
         if (c < ncon) {
             con(c).dln = dl[r][0];
@@ -823,7 +826,8 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
     pre.na = (int)(sizeof(float) * L.rows / sizeof(ContactTerms));
     pre.b = reinterpret_cast<ContactTerms *>(epa);
     pre.nb = (int)(NW * sizeof(EpaLds) / sizeof(ContactTerms));
-    solve_team<WARM>(t, Lb, n, DirectCon{con}, ncon, done, L.head, fl, a.k, Cf, Cr, pre, WARM ? wsum : nullptr);
+    solve_team<WARM, BATCH_SOLVE_REG>(t, Lb, n, DirectCon{con}, ncon, done, L.head, fl, a.k, Cf, Cr, pre,
+                                      WARM ? wsum : nullptr);
     if constexpr (WARM) {  // this step's list: the first warm_cap[w] contacts (contact c's sums are team thread c % T's)
         const int64_t off = a.warm_off[w];
         const int m = min(ncon, a.warm_cap[w]);
