@@ -146,10 +146,11 @@ struct WorldSrc {
     ContactRec *rec;
     int *flag;
     unsigned long long *cnt;
+    PairDecode32 dec;  // SMALL: the world's decode terms
     __device__ int key(int c) const { return cand[c]; }
     __device__ void shapes(int key, Shape &A, Shape &Bs) const
     {
-        PairId q = SMALL ? decode_pair32(key, nb, ns) : decode_pair(key, nb, ns);
+        PairId q = SMALL ? decode_pair32(key, dec) : decode_pair(key, nb, ns);
         int sa, sb;
         pair_slots(q, nb, sa, sb);
         A = body_shape(B, verts, sa, nb, *fl);
@@ -158,7 +159,7 @@ struct WorldSrc {
     static constexpr bool kMemCorners = true;
     __device__ void shapes_ref(int key, Shape &A, Shape &Bs) const
     {
-        PairId q = SMALL ? decode_pair32(key, nb, ns) : decode_pair(key, nb, ns);
+        PairId q = SMALL ? decode_pair32(key, dec) : decode_pair(key, nb, ns);
         int sa, sb;
         pair_slots(q, nb, sa, sb);
         A = body_shape_ref(B, verts, sa, nb, *fl);
@@ -169,7 +170,7 @@ struct WorldSrc {
     {
         if (r.status == EPA_OVERFLOW) atomicAdd(&cnt[CNT_EPA_OVERFLOW], 1ull);
         if (r.status != EPA_CONTACT) return;
-        PairId q = SMALL ? decode_pair32(key, nb, ns) : decode_pair(key, nb, ns);
+        PairId q = SMALL ? decode_pair32(key, dec) : decode_pair(key, nb, ns);
         ContactRec &o = rec[c];
         int sa, sb;
         pair_slots(q, nb, sa, sb);
@@ -469,7 +470,7 @@ __global__ __launch_bounds__(256) void k_plain_narrow(Planes B, const float *ver
     const int range = min(NARROW_RANGE, max(64, ((total + (int)gridDim.x - 1) / (int)gridDim.x + 63) & ~63));
     for (int base = blockIdx.x * range; base < total; base += gridDim.x * range) {
         const int count = min(range, total - base);
-        WorldSrc<false> src{B, verts, nb, ns, &fl, cand + base, rec + base, flag + base, cnt};
+        WorldSrc<false> src{B, verts, nb, ns, &fl, cand + base, rec + base, flag + base, cnt, PairDecode32{}};
         narrow_block<256, false>(src, count, np, lds, sh, qidx + base, qsim + base);
         if (threadIdx.x == 0) {
             atomicAdd(&cnt[CNT_GJK_FOUND], (unsigned long long)sh.nepa);
@@ -673,6 +674,7 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
     // narrowphase); the regions are then concatenated into cand[] in wave order
     // (a one-wave team writes cand[] directly)
     const int64_t npairs = pair_count(nb, ns);
+    const PairDecode32 dec = pair_decode32(nb, ns);
     const int wv = tid >> 6, lane = tid & 63;
     const int64_t share = (npairs + NW - 1) / NW;
     const int64_t q0 = min(npairs, (int64_t)wv * share), q1 = min(npairs, q0 + share);
@@ -685,7 +687,7 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
     const int ncc = nb * (nb - 1) / 2;
     int ci = 0, cj = 0;
     if (q0 < min(q1, (int64_t)ncc)) {
-        const PairId s0 = decode_pair32((int)q0, nb, ns);
+        const PairId s0 = decode_pair32((int)q0, dec);
         ci = __builtin_amdgcn_readfirstlane(s0.i);  // the same in every lane: kept scalar
         cj = __builtin_amdgcn_readfirstlane(s0.j);
     }
@@ -729,7 +731,7 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
             if (p < q1 && p < ncc) k = cull_keep(Lb.get3(F_PX, sa), bradL[sa], Lb.get3(F_PX, sb), bradL[sb]);
         }
         if (p >= ncc && p < q1) {
-            const PairId q = decode_pair32(p, nb, ns);
+            const PairId q = decode_pair32(p, dec);
             int sa, sb;
             pair_slots(q, nb, sa, sb);
             V3 ca, cb;
@@ -771,7 +773,7 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
     t.sync();
     // narrowphase over the candidates (GJK per lane, EPA per wave)
     {
-        WorldSrc<true> src{Lb, vertsL, nb, ns, &fl, cand, rec, flag, a.cnt};
+        WorldSrc<true> src{Lb, vertsL, nb, ns, &fl, cand, rec, flag, a.cnt, dec};
         narrow_team<false, ISL>(t, src, ncand, a.k.np, epa, nsh, S.qidx, S.qsim);
     }
     if (tid == 0) {

@@ -218,11 +218,41 @@ __device__ inline PairId decode_pair(int64_t p, int nb, int ns)
 
 // decode_pair for worlds whose pair count fits 24 bits (the batched path):
 // binary32 estimates of row and quotient (hardware sqrt and reciprocal) made
-// exact by integer fix-ups.
-__device__ inline void tri_decode32(int p, int n, int &i, int &j)
+// exact by integer fix-ups.  The terms that depend on the world's body counts
+// alone are computed once per world (pair_decode32, where every lane of the
+// wave runs) and made scalar, so that they cost no vector registers in the
+// loops that decode pairs.
+__device__ __forceinline__ float wave_uniform(float v)
 {
-    const float b = 2.0f * (float)n - 1.0f;
-    const float disc = b * b - 8.0f * (float)p;
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+struct PairDecode32 {
+    int nb, ns;
+    float bc, bbc;  // 2 nb - 1 and its square (box-box triangle)
+    float bs, bbs;  // 2 ns - 1 and its square (sphere-sphere triangle)
+    float rns;      // estimate of 1 / ns
+};
+
+// The body counts must be the same in every lane of the wave (one world per
+// wave or workgroup), and every lane must run this.
+__device__ inline PairDecode32 pair_decode32(int nb, int ns)
+{
+    PairDecode32 d;
+    d.nb = nb;
+    d.ns = ns;
+    d.bc = wave_uniform(2.0f * (float)nb - 1.0f);
+    d.bbc = wave_uniform(d.bc * d.bc);
+    d.bs = wave_uniform(2.0f * (float)ns - 1.0f);
+    d.bbs = wave_uniform(d.bs * d.bs);
+    d.rns = wave_uniform(__builtin_amdgcn_rcpf((float)ns));
+    return d;
+}
+
+// b = 2n - 1 and bb = b * b as binary32
+__device__ inline void tri_decode32(int p, int n, float b, float bb, int &i, int &j)
+{
+    const float disc = bb - 8.0f * (float)p;
     int r = (int)((b - __builtin_amdgcn_sqrtf(disc > 0.0f ? disc : 0.0f)) * 0.5f);
     r = r < 0 ? 0 : (r > n - 1 ? n - 1 : r);
     auto start = [n](int row) { return row * (2 * n - row - 1) / 2; };
@@ -232,13 +262,14 @@ __device__ inline void tri_decode32(int p, int n, int &i, int &j)
     j = p - start(r) + r + 1;
 }
 
-__device__ inline PairId decode_pair32(int p, int nb, int ns)
+__device__ inline PairId decode_pair32(int p, const PairDecode32 &d)
 {
+    const int nb = d.nb, ns = d.ns;
     PairId q;
     const int ncc = nb * (nb - 1) / 2;
     if (p < ncc) {
         q.type = NANS_CC;
-        tri_decode32(p, nb, q.i, q.j);
+        tri_decode32(p, nb, d.bc, d.bbc, q.i, q.j);
         return q;
     }
     p -= ncc;
@@ -256,7 +287,7 @@ __device__ inline PairId decode_pair32(int p, int nb, int ns)
     p -= ns;
     if (p < nb * ns) {
         q.type = NANS_CS;
-        int i = (int)((float)p * __builtin_amdgcn_rcpf((float)ns));
+        int i = (int)((float)p * d.rns);
         i = i < 0 ? 0 : i;
         while (i * ns > p) --i;
         while ((i + 1) * ns <= p) ++i;
@@ -266,7 +297,7 @@ __device__ inline PairId decode_pair32(int p, int nb, int ns)
     }
     p -= nb * ns;
     q.type = NANS_SS;
-    tri_decode32(p, ns, q.i, q.j);
+    tri_decode32(p, ns, d.bs, d.bbs, q.i, q.j);
     return q;
 }
 
