@@ -290,8 +290,9 @@ int nans_set_broadphase(nans_ctx *ctx, int32_t mode);
  * bodies; at 64 threads only 0), skipping combinations whose LDS does not fit or
  * that split the worlds as the previous bound did (ms = -1); it restores the
  * state, force/torque, quaternions and statistics, and keeps the fastest
- * combination for later nans_step_batched / nans_rollout calls.  Results do not
- * depend on either.  nans_set_batched_block forces a size (0: the default, 128
+ * combination for later nans_step_batched / nans_rollout calls, unless it is
+ * less than 2 % faster than the default (which then stays: a few timed steps do
+ * not rank shapes closer than that).  Results do not depend on either.  nans_set_batched_block forces a size (0: the default, 128
  * threads when no world holds more than 64 bodies, else 256);
  * nans_set_wave_worlds forces a bound (-1: the default, 0 = every world by a
  * whole workgroup; at most 512). */

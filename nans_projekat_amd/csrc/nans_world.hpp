@@ -453,7 +453,15 @@ __host__ __device__ inline void contact_apply(const Planes &B, ContactRec &c, co
         if (sumT2 > MaxLT) sumT2 = MaxLT;
     };
     if (k.solver_iterations > 0) {
-        for (int it = 1; it < k.solver_iterations; ++it) iterate();
+        // unrolled, so that an iteration's friction bound (from its normal sum
+        // alone) is computed while the friction sums of earlier ones still run
+        if (k.solver_iterations == 10) {
+#pragma unroll
+            for (int it = 1; it < 10; ++it) iterate();
+        } else {
+#pragma unroll 4
+            for (int it = 1; it < k.solver_iterations; ++it) iterate();
+        }
         const float oldN = sumN, oldT1 = sumT1, oldT2 = sumT2;
         iterate();
         dln = sumN - oldN;
