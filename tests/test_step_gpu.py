@@ -3,6 +3,7 @@
 lists and body state bit for bit, positions within 1e-3 m after 240 steps, and
 the error against the FP64 oracle reported."""
 import numpy as np
+import torch
 import pytest
 
 from nans_projekat_amd import runtime
@@ -216,4 +217,10 @@ def test_full_config3_properties(oracle_mod):
     golden = load_state_hash("config3_240_steps")
     assert golden["bodies"] == sc.nbodies and golden["steps"] == sc.nsteps
     assert state_hash(got) == golden["sha256"]
+    # the launch shape the step's speed rests on: 256 threads per world and three
+    # workgroups per CU (a 256-body world's LDS within a third of the CU's 160 KB)
+    t = ctx.batched_timing()
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    assert t["threads_per_world"] == 256 and t["lds_bytes"] <= 160 * 1024 // 3, t
+    assert t["workgroups"] == 3 * cus, (t, cus)
     ctx.close()
