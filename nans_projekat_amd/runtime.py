@@ -295,8 +295,9 @@ class Context:
 
     def autotune(self, nsteps: int = 5) -> dict:
         """Time the batched step at 64/128/256/512 threads per world and wave-world
-        bounds 0/16/32/64/128 bodies on the uploaded batch and keep the fastest
-        (nans_autotune_batched); the state is restored.  ms -1: skipped."""
+        bounds 0/16/32/64/128 bodies on the uploaded batch and keep the fastest,
+        unless it beats the default by less than 2 % (nans_autotune_batched); the
+        state is restored.  ms -1: skipped."""
         self._sync_stream()
         t = Tuning()
         check(self.lib.nans_autotune_batched(self.h, int(nsteps), ctypes.byref(t)))
