@@ -753,25 +753,33 @@ __device__ __forceinline__ void batched_world(const BatchArgsT<WARM> &a, const T
             woff[i] = ncand;
             ncand += i < NW ? wave_tot[i] : 0;
         }
-        // cand[c] for c < min(ncand, cap) from the region of the wave whose share
-        // holds c; a thread's copies are independent, so it issues two loads
-        // before their stores
-        const int nc = min(ncand, S.cap);
-        const int *wbase = reinterpret_cast<const int *>(rec);
-        auto src = [&](int c) {
-            int wi = 0, off = 0;
+        if constexpr (WARM || BLOCK > 256) {
+            // each wave's region in turn, one load at a time (the warm-started and the
+            // 512-thread kernels spill with the two-load copy below)
 #pragma unroll
-            for (int q = 1; q < NWMAX; ++q) {
-                if (q < NW && c >= woff[q]) {
-                    wi = q;
-                    off = woff[q];
-                }
+            for (int i = 0; i < NWMAX; ++i) {
+                if (i >= NW) break;
+                const int cnt = min(wave_tot[i], S.cap);
+                const int *src = reinterpret_cast<const int *>(rec) + (int64_t)i * S.cap;
+                for (int k = tid; k < cnt && woff[i] + k < S.cap; k += T) cand[woff[i] + k] = src[k];
             }
-            return wbase + (int64_t)wi * S.cap + (c - off);
-        };
-        if constexpr (WARM) {  // one load at a time: the warm-started kernel spills otherwise
-            for (int c = tid; c < nc; c += T) cand[c] = *src(c);
         } else {
+            // cand[c] for c < min(ncand, cap) from the region of the wave whose share
+            // holds c; a thread's copies are independent, so it issues two loads
+            // before their stores
+            const int nc = min(ncand, S.cap);
+            const int *wbase = reinterpret_cast<const int *>(rec);
+            auto src = [&](int c) {
+                int wi = 0, off = 0;
+#pragma unroll
+                for (int q = 1; q < NWMAX; ++q) {
+                    if (q < NW && c >= woff[q]) {
+                        wi = q;
+                        off = woff[q];
+                    }
+                }
+                return wbase + (int64_t)wi * S.cap + (c - off);
+            };
             for (int c = tid; c < nc; c += 2 * T) {
                 const int c2 = c + T;
                 const int v = *src(c);
