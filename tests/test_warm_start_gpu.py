@@ -113,6 +113,31 @@ def test_config3_worlds_batched_warm(oracle_mod, wave_bound):
     ctx.close()
 
 
+@pytest.mark.parametrize("threads", [64, 512])
+def test_config3_worlds_batched_warm_sizes(oracle_mod, threads):
+    """The warm-started batched step at 64 and 512 threads per world (instantiations
+    of their own) on six config-3 worlds, 40 steps, against the oracle's warm start."""
+    sc = S.config3(128, nsteps=1).subset([0, 17, 40, 77, 102, 127])  # 8 to 249 bodies
+    ctx = _warm_ctx(sc)
+    ctx.set_batched_block(threads)
+    ctx.step_batched(20)
+    ctx.step_batched(20)
+    assert ctx.batched_timing()["threads_per_world"] == threads
+    got = ctx.download()
+    st = ctx.stats()
+    assert st["epa_overflow"] == 0 and st["capacity_overflow"] == 0
+    for w in range(sc.nworlds):
+        ws = _oracle(oracle_mod, sc, w)
+        ws.run(39)
+        ref = ws.run(1, contacts=True)
+        _equal(got, ws, int(sc.offsets[w]), int(sc.offsets[w + 1]), "%d threads, world %d" % (threads, w))
+        con = ctx.contacts(w)
+        assert len(con) == len(ref), w
+        for f in ("index_a", "index_b", "dl_normal", "dl_tangent1", "dl_tangent2"):
+            np.testing.assert_array_equal(con[f], ref[f], err_msg="%d threads, world %d %s" % (threads, w, f))
+    ctx.close()
+
+
 def test_plain_and_batched_share_the_lists(oracle_mod):
     sc = S.config4(8)
     ctx = _warm_ctx(sc)
