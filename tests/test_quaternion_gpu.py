@@ -1,5 +1,6 @@
-"""The opt-in quaternion orientation on the GPU (plain and batched steps) against
-the oracle's quaternion mode, bit for bit; the Euler-to-quaternion conversion of
+"""The opt-in quaternion orientation on the GPU (plain and batched steps, the latter
+at several workgroup sizes and wave-world bounds) against the oracle's quaternion
+mode, bit for bit; the Euler-to-quaternion conversion of
 nans_set_orientation_mode; the differentiable rollout's refusal of the mode."""
 import ctypes
 
@@ -62,10 +63,18 @@ def test_config1_quaternion_mode_bit_exact(oracle_mod, path):
     ctx.close()
 
 
-def test_config3_worlds_quaternion_mode_batched(oracle_mod):
+# (threads per world, wave-world bound): the default shape, the 64- and 512-thread
+# instantiations, and the worlds of at most 16 bodies one per wave
+QUAT_SHAPES = ((0, -1), (64, -1), (512, 0), (256, 16))
+
+
+@pytest.mark.parametrize("threads,bound", QUAT_SHAPES)
+def test_config3_worlds_quaternion_mode_batched(oracle_mod, threads, bound):
     full = S.config3(256, nsteps=1)
     sc = full.subset(list(range(0, 256, 23)))
     ctx = _quat_ctx(sc, angles_seed=3)
+    ctx.set_batched_block(threads)
+    ctx.set_wave_worlds(bound)
     q0 = ctx.quaternions()
     ctx.step_batched(60)
     got, q = ctx.download(), ctx.quaternions()
