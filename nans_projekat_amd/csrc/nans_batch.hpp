@@ -46,6 +46,7 @@ struct BatchArgs {
     int wrows;              // LDS stride of a wave's world (>= largest of order[nbig, nworlds))
     int lds_floats;         // floats of world LDS before the polytope LDS (a multiple of 4)
     int *queue;             // this step's work counters: [0] workgroup worlds, [1] wave worlds
+    int *queue_next;        // the next step's counters, cleared by this launch (or null)
     StepConsts k;
     const FloorDev *fl;
     const float *wfric, *wrest;  // per-world coefficients (may be null)
@@ -214,7 +215,8 @@ struct nans_worlds {
     std::vector<int64_t> h_rec_off;
     std::vector<int> h_rec_cap;
     // graph cache of the batched step
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;   // two steps, on counter buffers A then B
+    hipGraphExec_t graph1 = nullptr;  // one step, on buffer A
     int graph_steps = -1;
     int graph_block = 0;
     nans_params graph_params;

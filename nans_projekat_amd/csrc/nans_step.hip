@@ -909,6 +909,11 @@ k_batched_step(BatchArgsT<WARM> a)
     const FloorDev &fl = *a.fl;  // in global memory, as the box corners
     __shared__ NarrowShared nsh[NW];
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    // the next launch's counters: no launch of this step reads them
+    if (a.queue_next && blockIdx.x == 0 && threadIdx.x == 0) {
+        a.queue_next[0] = 0;
+        a.queue_next[1] = 0;
+    }
     EpaLds *const epa = reinterpret_cast<EpaLds *>(smem + a.lds_floats);
     const int64_t wg = (int64_t)blockIdx.x * a.cap;
     // the workgroup's team, LDS and scratch until the first phase ends
@@ -973,7 +978,9 @@ void wfree(void *p)
 static void drop_graph(nans_worlds *W)
 {
     if (W->graph) (void)hipGraphExecDestroy(W->graph);
+    if (W->graph1) (void)hipGraphExecDestroy(W->graph1);
     W->graph = nullptr;
+    W->graph1 = nullptr;
     W->graph_steps = -1;
 }
 
@@ -1762,12 +1769,15 @@ static int batched_prepare(nans_ctx *ctx, BatchArgs &a, int *grid_out, size_t *l
         W->b_grid = grid;
         W->b_cap = (int)cap;
     }
-    if (W->queue_len < 2) {
+    // two buffers of work counters, A = queue[0, 2) and B = queue[2, 4), zero here;
+    // every call leaves A zero (B is cleared by the launch before the one using it)
+    if (W->queue_len < 4) {
         drop_graph(W);
         wfree(W->queue);
         W->queue = nullptr;
-        NANS_TRY(dmalloc(&W->queue, 2));
-        W->queue_len = 2;
+        NANS_TRY(dmalloc(&W->queue, 4));
+        NANS_HIP(hipMemset(W->queue, 0, sizeof(int) * 4));
+        W->queue_len = 4;
     }
     a.planes = W->planes;
     a.N = W->N;
@@ -1807,6 +1817,7 @@ static int batched_prepare(nans_ctx *ctx, BatchArgs &a, int *grid_out, size_t *l
     a.rec_cap = W->rec_cap;
     a.cnt = W->cnt;
     a.queue = W->queue;
+    a.queue_next = nullptr;
     if (split_out) *split_out = sp;
     *grid_out = grid;
     *lds_out = lds;
@@ -1822,18 +1833,29 @@ static int batched_capture(nans_ctx *ctx, int nsteps, hipStream_t cs)
     size_t lds = 0;
     WaveSplit sp;
     NANS_TRY((batched_prepare<BLOCK, false, ISL>(ctx, a, &grid, &lds, &sp)));
-    // one step: reset the work counters with a fill kernel (no memset node), then the
-    // persistent step kernel; a call of nsteps replays this graph nsteps times
-    hipGraph_t g = nullptr;
-    NANS_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    k_zero_i32<<<1, 64, 0, cs>>>(W->queue, 2);
-    k_batched_step<BLOCK, false, ISL><<<grid, BLOCK, lds, cs>>>(a);
-    hipError_t e = hipStreamEndCapture(cs, &g);
-    NANS_REQUIRE(e == hipSuccess, NANS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    // two steps: the step kernel on counter buffer A, clearing B, then on B, clearing
+    // A (no clearing kernel or memset node per step); and one step on A, after which
+    // the caller clears A.  A call of nsteps replays the first graph nsteps / 2 times
+    // and the second once if nsteps is odd.
+    int *const qa = W->queue, *const qb = W->queue + 2;
+    auto capture = [&](hipGraphExec_t *out, int steps) -> int {
+        hipGraph_t g = nullptr;
+        NANS_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < steps; ++k) {
+            a.queue = k ? qb : qa;
+            a.queue_next = steps == 1 ? nullptr : (k ? qa : qb);
+            k_batched_step<BLOCK, false, ISL><<<grid, BLOCK, lds, cs>>>(a);
+        }
+        hipError_t e = hipStreamEndCapture(cs, &g);
+        NANS_REQUIRE(e == hipSuccess, NANS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        NANS_REQUIRE(e == hipSuccess, NANS_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+        return NANS_OK;
+    };
     drop_graph(W);
-    e = hipGraphInstantiate(&W->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    NANS_REQUIRE(e == hipSuccess, NANS_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    NANS_TRY(capture(&W->graph, 2));
+    NANS_TRY(capture(&W->graph1, 1));
     W->graph_steps = nsteps;
     W->graph_block = BLOCK;
     W->graph_params = ctx->p;
@@ -1872,7 +1894,12 @@ int batched_run(nans_ctx *ctx, int nsteps, int block)
     NANS_HIP(hipStreamWaitEvent(cs, W->ev_in, 0));
     // events on the stream the kernels run on bracket the nsteps graph launches
     NANS_HIP(hipEventRecord(W->ev_t0, cs));
-    for (int s = 0; s < nsteps; ++s) NANS_HIP(hipGraphLaunch(W->graph, cs));
+    int s = 0;
+    for (; s + 2 <= nsteps; s += 2) NANS_HIP(hipGraphLaunch(W->graph, cs));
+    if (s < nsteps) {
+        NANS_HIP(hipGraphLaunch(W->graph1, cs));
+        k_zero_i32<<<1, 64, 0, cs>>>(W->queue, 2);  // A zero again for the next call
+    }
     NANS_HIP(hipEventRecord(W->ev_t1, cs));
     NANS_HIP(hipEventRecord(W->ev_out, cs));
     NANS_HIP(hipStreamWaitEvent(ctx->stream, W->ev_out, 0));
@@ -1901,8 +1928,8 @@ static int batched_warm_launches(nans_ctx *ctx, int nsteps, hipStream_t cs)
         a.warm_p_next = W->warm_p[nxt];
         a.warm_s_next = W->warm_s[nxt];
         a.warm_n_next = W->warm_n[nxt];
-        k_zero_i32<<<1, 64, 0, cs>>>(a.queue, 2);
         k_batched_step<BLOCK, true, ISL><<<grid, BLOCK, lds, cs>>>(a);
+        k_zero_i32<<<1, 64, 0, cs>>>(a.queue, 2);  // counter buffer A is zero between launches
         NANS_HIP(hipGetLastError());
         W->warm_flip = nxt;
     }
@@ -1965,8 +1992,8 @@ static int batched_recorded_launch(nans_ctx *ctx, hipStream_t s, ContactRec *rec
     a.rec_off = rec_off;
     a.rec_count = rec_count;
     a.rec_cap = rec_cap;
-    k_zero_i32<<<1, 64, 0, s>>>(a.queue, 2);
     k_batched_step<BLOCK, false, ISL><<<grid, BLOCK, lds, s>>>(a);
+    k_zero_i32<<<1, 64, 0, s>>>(a.queue, 2);  // counter buffer A is zero between launches
     NANS_HIP(hipGetLastError());
     return NANS_OK;
 }
